@@ -1,0 +1,54 @@
+"""Builds fvstuff_amd/libdgsem.so for gfx950 with hipcc (in-tree).
+
+One object per polynomial degree (dgsem_inst.hip with -DDG_P=p) plus the C-ABI
+unit, compiled in parallel, linked into one shared library.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libdgsem.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+DEGREES = range(1, 8)
+
+FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    # packed FP32 instructions are kept out of every kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout)
+    return r.stdout
+
+
+def build(jobs=None, verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    units = [(os.path.join(CSRC, "dgsem_inst.hip"), os.path.join(BUILD, f"dgsem_p{p}.o"), [f"-DDG_P={p}"])
+             for p in DEGREES]
+    units.append((os.path.join(CSRC, "dgsem_lib.hip"), os.path.join(BUILD, "dgsem_lib.o"), []))
+    cmds = [[HIPCC, *FLAGS, *extra, "-c", src, "-o", obj] for src, obj, extra in units]
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with ThreadPoolExecutor(jobs) as ex:
+        outs = list(ex.map(_run, cmds))
+    if verbose:
+        for o in outs:
+            if o.strip():
+                print(o)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + [u[1] for u in units])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

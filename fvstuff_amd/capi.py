@@ -1,0 +1,100 @@
+"""ctypes mirror of include/dgsem.h (structs, enums, status codes).
+
+Only type definitions live here; fvstuff_amd.dgsem loads the HIP library.
+"""
+import ctypes as C
+
+DG_OK = 0
+DG_ERR_INVALID_ARG = 1
+DG_ERR_HIP = 2
+DG_ERR_NONFINITE = 3
+DG_ERR_DT_UNDERFLOW = 4
+DG_ERR_BLOWUP = 5
+
+STATUS_NAMES = {
+    DG_OK: "ok",
+    DG_ERR_INVALID_ARG: "invalid argument",
+    DG_ERR_HIP: "HIP runtime error",
+    DG_ERR_NONFINITE: "non-finite wave speed",
+    DG_ERR_DT_UNDERFLOW: "dt underflow",
+    DG_ERR_BLOWUP: "blow-up",
+}
+
+VOL_CHANDRASHEKAR, VOL_RANOCHA, VOL_COLLOCATION = 0, 1, 2
+SURF_HLLC, SURF_RUSANOV = 0, 1
+BC_WALL, BC_PERIODIC = 0, 1
+FP64, MIXED = 0, 1
+IC_CIRCULAR_SOD, IC_VORTEX, IC_KELVIN_HELMHOLTZ = 0, 1, 2
+
+_NAMES = {
+    "volume_flux": {"chandrashekar": 0, "ranocha": 1, "collocation": 2},
+    "surface_flux": {"hllc": 0, "rusanov": 1},
+    "bc": {"wall": 0, "periodic": 1},
+    "precision": {"fp64": 0, "mixed": 1},
+}
+
+
+class DGConfig(C.Structure):
+    _fields_ = [
+        ("K", C.c_int), ("p", C.c_int),
+        ("x0", C.c_double), ("y0", C.c_double), ("L", C.c_double),
+        ("gamma", C.c_double),
+        ("volume_flux", C.c_int), ("surface_flux", C.c_int), ("bc", C.c_int),
+        ("av", C.c_int), ("limiter", C.c_int), ("precision", C.c_int),
+        ("av_cav", C.c_double), ("av_ksensor", C.c_double),
+        ("eps_rho", C.c_double), ("eps_p", C.c_double),
+        ("elems_per_block", C.c_int), ("xcd_remap", C.c_int),
+    ]
+
+
+class DGICParams(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("rho_hi", C.c_double), ("p_hi", C.c_double),
+        ("vortex_eps", C.c_double), ("xc", C.c_double), ("yc", C.c_double),
+        ("u0", C.c_double), ("v0", C.c_double),
+        ("seed", C.c_ulonglong), ("noise_amp", C.c_double),
+    ]
+
+
+def make_config(K=50, p=3, x0=0.0, y0=0.0, L=1.0, gamma=1.4,
+                volume_flux="chandrashekar", surface_flux="hllc", bc="wall",
+                av=True, limiter=True, precision="fp64", av_cav=0.25,
+                av_ksensor=0.1, eps_rho=1e-6, eps_p=1e-6,
+                elems_per_block=0, xcd_remap=-1):
+    """A dg_config; defaults are the reference's compile-time defaults
+    (dgsem_lobatto_2d.cu:61-126, Makefile DGSEM_ORDER=3, main :915-922)."""
+    def enum(field, v):
+        return _NAMES[field][v] if isinstance(v, str) else int(v)
+    return DGConfig(int(K), int(p), float(x0), float(y0), float(L), float(gamma),
+                    enum("volume_flux", volume_flux), enum("surface_flux", surface_flux),
+                    enum("bc", bc), int(bool(av)), int(bool(limiter)),
+                    enum("precision", precision), float(av_cav), float(av_ksensor),
+                    float(eps_rho), float(eps_p), int(elems_per_block), int(xcd_remap))
+
+
+def make_ic(kind="vortex", rho_hi=11.0, p_hi=10.0, vortex_eps=5.0, xc=5.0, yc=5.0,
+            u0=1.0, v0=1.0, seed=1234, noise_amp=1e-3):
+    kinds = {"sod": IC_CIRCULAR_SOD, "vortex": IC_VORTEX, "kh": IC_KELVIN_HELMHOLTZ}
+    k = kinds[kind] if isinstance(kind, str) else int(kind)
+    return DGICParams(k, float(rho_hi), float(p_hi), float(vortex_eps), float(xc), float(yc),
+                      float(u0), float(v0), int(seed), float(noise_amp))
+
+
+def vortex_config(K=32, p=3, precision="fp64", **kw):
+    """Isentropic vortex on periodic [0,10]^2 with the Ranocha EC volume flux
+    and Rusanov faces, no AV / limiter (they are inactive on smooth data)."""
+    args = dict(K=K, p=p, x0=0.0, y0=0.0, L=10.0, volume_flux="ranocha",
+                surface_flux="rusanov", bc="periodic", av=False, limiter=False,
+                precision=precision)
+    args.update(kw)
+    return make_config(**args)
+
+
+def kh_config(K=64, p=3, precision="mixed", **kw):
+    """Kelvin-Helmholtz step shear layer on periodic [-1,1]^2."""
+    args = dict(K=K, p=p, x0=-1.0, y0=-1.0, L=2.0, volume_flux="ranocha",
+                surface_flux="rusanov", bc="periodic", av=False, limiter=False,
+                precision=precision)
+    args.update(kw)
+    return make_config(**args)

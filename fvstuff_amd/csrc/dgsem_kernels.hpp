@@ -1,0 +1,578 @@
+// DGSEM stage kernels for gfx950.
+//
+// One workgroup owns E consecutive elements (a tile of one element row when
+// K >= E).  The work is split in three phases that hand data over through LDS:
+//
+//  A  node phase  - each thread loads whole nodes (4 coalesced FP64 streams,
+//                   the reference's var-major SoA, dgsem_lobatto_2d.cu:126),
+//                   applies cons_to_prim_sane in FP64 and stores
+//                   (rho, u, v, p, rho/p) in the compute precision.
+//  B  line phase  - one thread per grid line of an element (p+1 x-lines and
+//                   p+1 y-lines).  The thread keeps the line's p+1 states in
+//                   registers and evaluates every two-point volume flux of the
+//                   line once: F#(U_i,U_k) is symmetric, so each pair feeds
+//                   D_ik F# to node i and D_ki F# to node k (p(p+1)/2 flux
+//                   evaluations per line instead of the reference's (p+1)^2
+//                   per line, :417-430).  y-lines work on states with u and v
+//                   swapped so that every lane runs the same x-direction code
+//                   (no divergence between x- and y-lines).  The thread also
+//                   evaluates the Riemann flux on the two faces its line ends
+//                   on (:441-481) and the line's share of the artificial
+//                   viscosity (:501-521).
+//  C  node phase  - RHS per node = x-line part + y-line part; either written
+//                   out (RHS mode) or fused into the SSP-RK3 Shu-Osher update
+//                   (:533-576) with sanitize_cons, the Zhang-Shu limiter
+//                   (:584-643, FP64) and the wave-speed reduction for the next
+//                   step's dt (:483-486, :646-659).
+#pragma once
+#include "dgsem_physics.hpp"
+
+namespace dg {
+
+enum { MODE_RHS = 0, MODE_STAGE = 1 };
+enum { VOL_CHANDRASHEKAR = 0, VOL_RANOCHA = 1, VOL_COLLOCATION = 2 };
+enum { SURF_HLLC = 0, SURF_RUSANOV = 1 };
+enum { BC_WALL = 0, BC_PERIODIC = 1 };
+enum { ST_OK = 0, ST_NONFINITE = 3, ST_DT_UNDERFLOW = 4, ST_BLOWUP = 5 };
+
+constexpr int BLOCK = 256;
+
+// Device-resident step state for the fused stepping (no host round trip).
+struct DevState {
+    unsigned long long lam_bits[2];  // max wave speed as ordered bit pattern (>= 0)
+    double t;                        // simulation time
+    double dt;                       // dt of the current step
+    int status;                      // first numerical failure (ST_*)
+    int skip;                        // current step is a no-op (t >= t_end)
+    int pad[2];
+};
+
+struct KArgs {
+    const double* Up;    // state whose RHS is evaluated
+    const double* U0;    // U^n for the Shu-Osher combination
+    double* out;         // RHS (MODE_RHS) or new stage state (MODE_STAGE)
+    double* lam_el;      // MODE_RHS: per-element max wave speed (nullable)
+    const void* tab;     // compute-precision tables: D[N*N], w[N], winv[N]
+    const double* w64;   // FP64 GLL weights (limiter)
+    DevState* st;
+    int K, NEL, E, xcd;
+    int mode, stage, vol, surf, bc, av, limiter, want_lambda, slot_cur, slot_next;
+    double h, gamma, ca, cb, cfl, t_end, av_cav, av_k, eps_rho, eps_p;
+    int off_U, off_av, off_lam, off_el;   // byte offsets of the LDS regions
+};
+
+__device__ __forceinline__ unsigned long long dbits(double x) {
+    return (unsigned long long)__double_as_longlong(x);
+}
+__device__ __forceinline__ double bitsd(unsigned long long b) {
+    return __longlong_as_double((long long)b);
+}
+
+// XCD-aware tile order: the hardware deals workgroups round-robin over the 8
+// XCDs; give each XCD a contiguous range of tiles so that the y-neighbours a
+// tile reads (K/E tiles away) sit in the same XCD's L2.
+__device__ __forceinline__ int tile_of_block(int b, int G, int xcd) {
+    if (!xcd) return b;
+    int q = G >> 3, r = G & 7;
+    int x = b & 7, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
+// max that propagates NaN (fmax would drop it)
+__device__ __forceinline__ double nanmax(double a, double b) {
+    return (a != a || b != b) ? a + b : fmax(a, b);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block-wide max of v (FP64) -> one atomicMax on the bit pattern.
+__device__ __forceinline__ void block_max_atomic(double v, unsigned long long* dst, double* sred) {
+    v = wave_max(v);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sred[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = sred[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = nanmax(m, sred[i]);
+        // a positive NaN orders above every finite value as an unsigned bit pattern
+        if (m != m) m = __longlong_as_double(0x7ff8000000000000ll);
+        atomicMax(dst, dbits(m));
+    }
+}
+
+// ── Zhang-Shu limiter on E elements whose stage values sit in LDS (FP64) ──
+// sU[e][q][nd]; on return the limited values are in sU.  :584-643.
+template <int N>
+__device__ void limit_block(double* sU, const double* w64, int nE, double g, double eps_rho, double eps_p,
+                            double* sBar, double* sTrho, unsigned long long* sTheta) {
+    constexpr int N2 = N * N;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    __syncthreads();
+    if (tid < nE) {
+        const int e = tid;
+        const double* U = sU + (size_t)e * 4 * N2;
+        double Ubar[4] = {0, 0, 0, 0};
+        for (int jn = 0; jn < N; jn++)
+            for (int in = 0; in < N; in++) {
+                double wij = 0.25 * w64[in] * w64[jn];
+                for (int q = 0; q < 4; q++) Ubar[q] += wij * U[q * N2 + jn * N + in];
+            }
+        Ubar[0] = fmax(Ubar[0], eps_rho);
+        {
+            double p = Cons64::pressure_from_cons(Ubar, g);
+            if (p < eps_p) Ubar[3] = eps_p / (g - 1.0) + 0.5 * (Ubar[1] * Ubar[1] + Ubar[2] * Ubar[2]) / Ubar[0];
+        }
+        double rho_min = 1e30;
+        for (int n = 0; n < N2; n++) rho_min = fmin(rho_min, U[n]);
+        double t_rho = 1.0;
+        if (rho_min < eps_rho) t_rho = (Ubar[0] - eps_rho) / fmax(Ubar[0] - rho_min, 1e-30);
+        t_rho = fmax(0.0, fmin(1.0, t_rho));
+        for (int q = 0; q < 4; q++) sBar[e * 4 + q] = Ubar[q];
+        sTrho[e] = t_rho;
+        sTheta[e] = dbits(t_rho);
+    }
+    __syncthreads();
+    for (int s = tid; s < nE * N2; s += nthr) {
+        const int e = s / N2, nd = s - e * N2;
+        const double* U = sU + (size_t)e * 4 * N2;
+        double t_rho = sTrho[e];
+        double Ub[4], Un[4], Ud[4];
+        for (int q = 0; q < 4; q++) { Ub[q] = sBar[e * 4 + q]; Un[q] = U[q * N2 + nd]; }
+        for (int q = 0; q < 4; q++) Ud[q] = Ub[q] + t_rho * (Un[q] - Ub[q]);
+        if (Cons64::pressure_from_cons(Ud, g) < eps_p) {
+            double lo = 0.0, hi = t_rho;
+            for (int it = 0; it < 20; it++) {
+                double tm = 0.5 * (lo + hi), Um[4];
+                for (int q = 0; q < 4; q++) Um[q] = Ub[q] + tm * (Un[q] - Ub[q]);
+                if (Cons64::pressure_from_cons(Um, g) >= eps_p) lo = tm; else hi = tm;
+            }
+            atomicMin(&sTheta[e], dbits(lo));   // theta in [0,1]: bit order = value order
+        }
+    }
+    __syncthreads();
+    for (int s = tid; s < nE * N2; s += nthr) {
+        const int e = s / N2, nd = s - e * N2;
+        double* U = sU + (size_t)e * 4 * N2;
+        double th = bitsd(sTheta[e]);
+        for (int q = 0; q < 4; q++) {
+            double ub = sBar[e * 4 + q];
+            U[q * N2 + nd] = ub + th * (U[q * N2 + nd] - ub);
+        }
+    }
+    __syncthreads();
+}
+
+// Load node `nd` of element `el` (4 FP64 streams) and convert.
+__device__ __forceinline__ void load_prim64(const double* __restrict__ Q, size_t S, size_t off, double g,
+                                            double W[4]) {
+    double U[4] = {Q[off], Q[S + off], Q[2 * S + off], Q[3 * S + off]};
+    Cons64::to_prim(U, W, g);
+}
+
+template <typename Real>
+__device__ __forceinline__ void rot(Prim<Real>& w, bool swap) {
+    Real u = w.u, v = w.v;
+    w.u = swap ? v : u;
+    w.v = swap ? u : v;
+}
+
+// ── volume contribution of one line: acc[i] = sum_k D_ik F#(U_i,U_k) ──────
+template <int N, typename Real, int VOL>
+__device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Real> (&w)[N], const Real* D,
+                                            Real (&acc)[N][4]) {
+    if (VOL == VOL_COLLOCATION) {
+        // linear contraction sum_k D_ik f(U_k) (:431-439)
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            Real f[4];
+            ph.template flux<0>(w[k], f);
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const Real dik = D[i * N + k];
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[i][q] += dik * f[q];
+            }
+        }
+        return;
+    }
+    // diagonal: F#(U_i,U_i) = f(U_i) (consistency)
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        Real f[4];
+        ph.template flux<0>(w[i], f);
+        const Real dii = D[i * N + i];
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[i][q] += dii * f[q];
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+#pragma unroll
+        for (int k = i + 1; k < N; k++) {
+            Real F[4];
+            if (VOL == VOL_RANOCHA) ph.ranocha(w[i], w[k], F);
+            else ph.chandrashekar(w[i], w[k], F);
+            const Real dik = D[i * N + k], dki = D[k * N + i];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                acc[i][q] += dik * F[q];
+                acc[k][q] += dki * F[q];
+            }
+        }
+    }
+}
+
+// Register budget per precision: FP32 lines keep (p+1)*9 floats live, FP64
+// twice that; the occupancy floor keeps the scheduler from trading waves for
+// instruction-level parallelism beyond what the line needs.
+template <typename Real> struct Occ;
+template <> struct Occ<float> { static constexpr int waves = 3; };
+template <> struct Occ<double> { static constexpr int waves = 2; };
+
+template <int P, typename Real, bool FULL, int VOL>
+__global__ __launch_bounds__(BLOCK, Occ<Real>::waves) void stage_kernel(KArgs a) {
+    constexpr int N = P + 1, N2 = N * N;
+    using Mx = M<Real>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int G = gridDim.x;
+    const int tile = tile_of_block(blockIdx.x, G, a.xcd);
+    const int E = a.E;
+    const int el0 = tile * E;
+    const int nE = min(E, a.NEL - el0);
+    const size_t S = (size_t)a.NEL * N2;
+    const double g = a.gamma;
+
+    const Real* D = (const Real*)a.tab;
+    const Real* winv = D + N * N + N;
+
+    // ── fused dt (device side, :1026-1035) ──
+    double dt = 0.0;
+    if (a.mode == MODE_STAGE) {
+        DevState* st = a.st;
+        if (st->status != ST_OK) return;
+        if (a.stage == 1) {
+            double lam = bitsd(st->lam_bits[a.slot_cur]);
+            double t = st->t;
+            int status = ST_OK, skip = 0;
+            if (!(lam > 0.0)) status = ST_NONFINITE;
+            else if (lam > 1e6) status = ST_BLOWUP;
+            else {
+                if (a.t_end > 0.0 && t >= a.t_end - 1e-12 * fmax(1.0, fabs(a.t_end))) skip = 1;
+                dt = a.cfl * a.h / (lam * (double)N);
+                if (a.t_end > 0.0 && t + dt > a.t_end) dt = a.t_end - t;
+                if (!skip && dt < 1e-14) status = ST_DT_UNDERFLOW;
+            }
+            if (blockIdx.x == 0 && tid == 0) {
+                st->dt = (status == ST_OK && !skip) ? dt : 0.0;
+                st->skip = skip;
+                if (status != ST_OK) st->status = status;
+            }
+            if (status != ST_OK || skip) return;
+        } else {
+            if (st->skip) return;
+            dt = st->dt;
+            if (a.stage == 2 && blockIdx.x == 0 && tid == 0) {
+                st->t += dt;
+                st->lam_bits[a.slot_next] = 0ull;
+            }
+        }
+    }
+
+    const Phys<Real> ph{(Real)g, (Real)(g - 1.0), (Real)(1.0 / (g - 1.0))};
+
+    // LDS regions
+    Real* sW = (Real*)smem;                          // [E][N2][5] prims; reused as sR [E][4][N2]
+    Real* sR = sW;
+    double* sU = (double*)(smem + a.off_U);         // [E][4][N2] FP64 stage values (limiter)
+    Real* sUc = (Real*)(smem + a.off_av);           // [E][4][N2] AV: sanitized cons
+    Real* sDu = sUc + (size_t)E * 4 * N2;           // [E][N2]
+    Real* sDv = sDu + (size_t)E * N2;               // [E][N2]
+    Real* sNu = sDv + (size_t)E * N2;               // [E]
+    double* sLam = (double*)(smem + a.off_lam);     // [E][N2]
+    double* sEl = (double*)(smem + a.off_el);       // per-element scratch
+    double* sBar = sEl;                              // [E][4]
+    double* sTrho = sEl + 4 * E;                     // [E]
+    unsigned long long* sTheta = (unsigned long long*)(sEl + 5 * E);  // [E]
+    double* sred = sEl + 6 * E;                      // [BLOCK/64]
+
+    const bool need_lam = FULL && (a.av || (a.mode == MODE_RHS && a.lam_el != nullptr));
+
+    // ── A: load nodes, prims in FP64, store compute-precision prims ──
+    for (int s = tid; s < E * N2; s += nthr) {
+        const int e = s / N2, nd = s - e * N2;
+        double W[4] = {1.0, 0.0, 0.0, 1.0};
+        if (e < nE) load_prim64(a.Up, S, (size_t)(el0 + e) * N2 + nd, g, W);
+        Prim<Real> w = make_prim<Real>(W);
+        Real* p = sW + ((size_t)e * N2 + nd) * 5;
+        p[0] = w.r; p[1] = w.u; p[2] = w.v; p[3] = w.p; p[4] = w.z;
+        if (FULL) {
+            if (need_lam) sLam[e * N2 + nd] = Cons64::wave_speed(W, g);
+            if (a.av) {
+                Real* c = sUc + (size_t)e * 4 * N2 + nd;
+                c[0] = w.r; c[N2] = w.r * w.u; c[2 * N2] = w.r * w.v;
+                c[3 * N2] = w.p * ph.inv_gm1 + (Real)0.5 * w.r * (w.u * w.u + w.v * w.v);
+            }
+        }
+    }
+    __syncthreads();
+
+    const int nlines = E * 2 * N;
+    const bool is_line = tid < nlines;
+    const int le = is_line ? tid / (2 * N) : 0;       // element of this line
+    const int ll = is_line ? tid - le * 2 * N : 0;
+    const bool ydir = ll >= N;                          // y-line (u,v swapped)
+    const int lidx = ydir ? ll - N : ll;
+    const int lbase = ydir ? lidx : lidx * N;           // node index of the line's k-th node:
+    const int lstride = ydir ? N : 1;                   //   lbase + k*lstride
+    const bool line_live = is_line && le < nE;
+    const Real jac = (Real)(2.0 / a.h);
+
+    // ── AV pre-pass: velocity divergence, sensor, element viscosity (:483-521) ──
+    if (FULL && a.av) {
+        if (is_line) {
+            Real un[N];
+#pragma unroll
+            for (int k = 0; k < N; k++) un[k] = sW[((size_t)le * N2 + lbase + k * lstride) * 5 + (ydir ? 2 : 1)];
+            Real* dst = ydir ? sDv : sDu;
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                Real d = 0;
+#pragma unroll
+                for (int k = 0; k < N; k++) d += D[i * N + k] * un[k];
+                dst[le * N2 + lbase + i * lstride] = d;
+            }
+        }
+        __syncthreads();
+        if (tid < E) {
+            const int e = tid;
+            const Real len_p = (Real)(a.h / (2 * P + 1));
+            const Real dx2inv = (Real)1 / (len_p * len_p);
+            Real th_e = 0;
+            double lam_e = 0.0;
+            for (int nd = 0; nd < N2; nd++) {
+                const Real* p = sW + ((size_t)e * N2 + nd) * 5;
+                Real rho = Mx::fmax_(p[0], (Real)1e-12), c2 = ph.g * p[3] / rho;
+                Real divu = Mx::fmin_(jac * (sDu[e * N2 + nd] + sDv[e * N2 + nd]), (Real)0), du2 = divu * divu;
+                Real th = du2 / (du2 + (Real)a.av_k * c2 * dx2inv + (Real)1e-30);
+                th_e = Mx::fmax_(th_e, th);
+                lam_e = fmax(lam_e, sLam[e * N2 + nd]);
+            }
+            sNu[e] = (Real)a.av_cav * th_e * len_p * (Real)lam_e;
+        }
+        __syncthreads();
+    }
+
+    // ── B: line phase ──
+    Real acc[N][4];
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[i][q] = 0;
+
+    if (is_line) {
+        Prim<Real> w[N];
+        const Real* eb = sW + (size_t)le * N2 * 5;
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const Real* p = eb + (lbase + k * lstride) * 5;
+            w[k].r = p[0]; w[k].u = p[1]; w[k].v = p[2]; w[k].p = p[3]; w[k].z = p[4];
+            rot(w[k], ydir);
+        }
+        line_volume<N, Real, VOL>(ph, w, D, acc);
+        const Real scale = VOL == VOL_COLLOCATION ? -jac : (Real)-2 * jac;
+#pragma unroll
+        for (int i = 0; i < N; i++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[i][q] *= scale;
+
+        // ── faces at both ends of the line (one call site for the Riemann flux) ──
+        if (line_live) {
+            const int el = el0 + le;
+            const int ej = el / a.K, ei = el - ej * a.K;
+            const int cpos = ydir ? ej : ei;
+            Real cm[4], cp[4];
+#pragma unroll 1
+            for (int f = 0; f < 2; f++) {
+                const Prim<Real> own = f ? w[N - 1] : w[0];
+                const bool edge = f ? (cpos == a.K - 1) : (cpos == 0);
+                Prim<Real> nb;
+                if (edge && a.bc == BC_WALL) {
+                    nb = wall_ghost(own, 0);
+                } else {
+                    int c2 = f ? cpos + 1 : cpos - 1;
+                    if (edge) c2 = f ? 0 : a.K - 1;
+                    const int elnb = ydir ? c2 * a.K + ei : ej * a.K + c2;
+                    const int ndnb = lbase + (f ? 0 : N - 1) * lstride;
+                    double W[4];
+                    load_prim64(a.Up, S, (size_t)elnb * N2 + ndnb, g, W);
+                    nb = make_prim<Real>(W);
+                    rot(nb, ydir);
+                }
+                const Prim<Real> L = f ? own : nb, R = f ? nb : own;
+                Real Fs[4], fo[4];
+                if (a.surf == SURF_RUSANOV) ph.rusanov(0, L, R, Fs);
+                else ph.hllc(0, L, R, Fs);
+                ph.template flux<0>(own, fo);
+                const Real c = (f ? -jac : jac) * winv[f ? N - 1 : 0];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    Real v = c * (Fs[q] - fo[q]);
+                    if (f) cp[q] = v; else cm[q] = v;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) { acc[0][q] += cm[q]; acc[N - 1][q] += cp[q]; }
+        }
+
+        // ── artificial viscosity along the line: -jac W^-1 D^T W (nu jac D Uc) ──
+        if (FULL && a.av) {
+            const Real nu = sNu[le];
+            const Real* wq = D + N * N;
+            Real uc[N][4];
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                const Real* c = sUc + (size_t)le * 4 * N2 + lbase + k * lstride;
+                uc[k][0] = c[0];
+                uc[k][1] = ydir ? c[2 * N2] : c[N2];
+                uc[k][2] = ydir ? c[N2] : c[2 * N2];
+                uc[k][3] = c[3 * N2];
+            }
+            Real gr[N][4];
+#pragma unroll
+            for (int i = 0; i < N; i++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    Real s = 0;
+#pragma unroll
+                    for (int k = 0; k < N; k++) s += D[i * N + k] * uc[k][q];
+                    gr[i][q] = nu * jac * s;
+                }
+#pragma unroll
+            for (int i = 0; i < N; i++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    Real s = 0;
+#pragma unroll
+                    for (int k = 0; k < N; k++) s += wq[k] * D[k * N + i] * gr[k][q];
+                    acc[i][q] -= jac * winv[i] * s;
+                }
+        }
+        // rotate back: swap momentum components of y-lines
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            Real m1 = acc[i][1], m2 = acc[i][2];
+            acc[i][1] = ydir ? m2 : m1;
+            acc[i][2] = ydir ? m1 : m2;
+        }
+    }
+    __syncthreads();   // every read of sW is done: reuse it as sR
+    if (is_line && !ydir) {
+        Real* r = sR + (size_t)le * 4 * N2;
+#pragma unroll
+        for (int i = 0; i < N; i++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) r[q * N2 + lbase + i * lstride] = acc[i][q];
+    }
+    __syncthreads();
+    if (is_line && ydir) {
+        Real* r = sR + (size_t)le * 4 * N2;
+#pragma unroll
+        for (int i = 0; i < N; i++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) r[q * N2 + lbase + i * lstride] += acc[i][q];
+    }
+    __syncthreads();
+
+    // ── C: node phase ──
+    if (a.mode == MODE_RHS) {
+        for (int s = tid; s < nE * N2; s += nthr) {
+            const int e = s / N2, nd = s - e * N2;
+            const size_t o = (size_t)(el0 + e) * N2 + nd;
+            const Real* r = sR + (size_t)e * 4 * N2 + nd;
+#pragma unroll
+            for (int q = 0; q < 4; q++) a.out[q * S + o] = (double)r[q * N2];
+        }
+        if (FULL && a.lam_el != nullptr && tid < nE) {
+            double lam_e = 0.0;
+            for (int nd = 0; nd < N2; nd++) lam_e = fmax(lam_e, sLam[tid * N2 + nd]);
+            a.lam_el[el0 + tid] = lam_e;
+        }
+        return;
+    }
+
+    // MODE_STAGE: U_s = ca*U0 + cb*(Up + dt*R), sanitize, [limit], write, [lambda]
+    const bool lim = FULL && a.limiter;
+    double lmax = 0.0;
+    for (int s = tid; s < nE * N2; s += nthr) {
+        const int e = s / N2, nd = s - e * N2;
+        const size_t o = (size_t)(el0 + e) * N2 + nd;
+        const Real* r = sR + (size_t)e * 4 * N2 + nd;
+        double Us[4];
+        if (a.ca != 0.0) {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                Us[q] = a.ca * a.U0[q * S + o] + a.cb * (a.Up[q * S + o] + dt * (double)r[q * N2]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++) Us[q] = a.Up[q * S + o] + dt * (double)r[q * N2];
+        }
+        Cons64::sanitize_cons(Us, g);
+        if (lim) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) sU[(size_t)e * 4 * N2 + q * N2 + nd] = Us[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++) a.out[q * S + o] = Us[q];
+            if (a.want_lambda) {
+                double W[4];
+                Cons64::to_prim(Us, W, g);
+                lmax = nanmax(lmax, Cons64::wave_speed(W, g));
+            }
+        }
+    }
+    if (lim) {
+        limit_block<N>(sU, a.w64, nE, g, a.eps_rho, a.eps_p, sBar, sTrho, sTheta);
+        for (int s = tid; s < nE * N2; s += nthr) {
+            const int e = s / N2, nd = s - e * N2;
+            const size_t o = (size_t)(el0 + e) * N2 + nd;
+            double Us[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) { Us[q] = sU[(size_t)e * 4 * N2 + q * N2 + nd]; a.out[q * S + o] = Us[q]; }
+            if (a.want_lambda) {
+                double W[4];
+                Cons64::to_prim(Us, W, g);
+                lmax = nanmax(lmax, Cons64::wave_speed(W, g));
+            }
+        }
+    }
+    if (a.want_lambda) block_max_atomic(lmax, &a.st->lam_bits[a.slot_next], sred);
+}
+
+// Host-side LDS layout shared by the launcher and the kernel.
+struct LdsLayout {
+    int off_U, off_av, off_lam, off_el, total;
+};
+
+inline int align_up(int x, int a) { return (x + a - 1) / a * a; }
+
+inline LdsLayout lds_layout(int N, int E, int real_bytes, bool limiter, bool av, bool lam) {
+    const int N2 = N * N;
+    LdsLayout L;
+    int off = align_up(E * N2 * 5 * real_bytes, 16);
+    L.off_U = off;
+    if (limiter) off += align_up(E * 4 * N2 * 8, 16);
+    L.off_av = off;
+    if (av) off += align_up((E * 4 * N2 + 2 * E * N2 + E) * real_bytes, 16);
+    L.off_lam = off;
+    if (lam || av) off += align_up(E * N2 * 8, 16);
+    L.off_el = off;
+    off += align_up((6 * E + BLOCK / 64) * 8, 16);
+    L.total = off;
+    return L;
+}
+
+}  // namespace dg

@@ -1,0 +1,33 @@
+// Host launchers for the per-degree kernel instantiations (one object file per p).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "dgsem_kernels.hpp"
+
+namespace dg {
+
+// Launches stage_kernel<P, Real, FULL>.  Returns the launch's hipError_t.
+typedef hipError_t (*StageLauncher)(const KArgs& a, bool fp32, bool full, int grid, int block,
+                                    size_t lds, hipStream_t s);
+// Limiter on E-element tiles in place.
+typedef hipError_t (*LimitLauncher)(double* Q, const double* w64, int NEL, int E, double g, double eps_rho,
+                                    double eps_p, hipStream_t s);
+// Per-element shock sensor.
+typedef hipError_t (*SensorLauncher)(const double* Q, double* sensor, const double* D64, int K, double h,
+                                     double g, double ksensor, hipStream_t s);
+
+#define DG_DECLARE_P(P)                                                                              \
+    hipError_t launch_stage_p##P(const KArgs&, bool, bool, int, int, size_t, hipStream_t);          \
+    hipError_t launch_limit_p##P(double*, const double*, int, int, double, double, double, hipStream_t); \
+    hipError_t launch_sensor_p##P(const double*, double*, const double*, int, double, double, double, hipStream_t);
+DG_DECLARE_P(1)
+DG_DECLARE_P(2)
+DG_DECLARE_P(3)
+DG_DECLARE_P(4)
+DG_DECLARE_P(5)
+DG_DECLARE_P(6)
+DG_DECLARE_P(7)
+#undef DG_DECLARE_P
+
+constexpr int PMAX_GPU = 7;
+
+}  // namespace dg

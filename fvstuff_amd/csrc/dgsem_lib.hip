@@ -1,0 +1,609 @@
+// C ABI of the MI355X DGSEM step (include/dgsem.h) and the small kernels that
+// are not per-degree: IC, unfused RK stage, wave-speed reductions, totals.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <vector>
+
+#include "../../include/dgsem.h"
+#include "dgsem_launch.hpp"
+
+using namespace dg;
+
+struct dg_ctx {
+    dg_config cfg;
+    hipStream_t stream;
+    int N, N2, NEL;
+    double h;
+    double x[16], w[16], D[256];        // host FP64 operator tables
+    void* d_tab64 = nullptr;             // D, w, winv in FP64
+    void* d_tab32 = nullptr;             // D, w, winv in FP32
+    double* d_U1 = nullptr;              // stage buffers of the fused stepping
+    double* d_U2 = nullptr;
+    DevState* d_st = nullptr;
+    unsigned long long* d_red = nullptr; // reduction slot for dg_cfl_dt
+    double* d_part = nullptr;            // per-block partial sums for dg_totals
+    double* d_xi = nullptr;              // LGL node coordinates (IC kernel)
+    int part_blocks = 0;
+    int E = 0, xcd = 0;
+    char err[512] = {0};
+};
+
+// ── error plumbing ─────────────────────────────────────────────────────────
+static int fail(dg_ctx* c, int code, const char* fmt, ...) {
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof(c->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+#define HIPCK(c, x)                                                                         \
+    do {                                                                                    \
+        hipError_t _e = (x);                                                                \
+        if (_e != hipSuccess)                                                               \
+            return fail((c), DG_ERR_HIP, "%s:%d: %s", __FILE__, __LINE__, hipGetErrorString(_e)); \
+    } while (0)
+
+// ── operators: LGL nodes / GLL weights and D (replaces :145-179, :924-940) ──
+// p <= 3 use the reference's literal tables (dgsem_lobatto_2d.cu:145-157);
+// higher p: x_j are +-1 and the zeros of L_{p+1} - L_{p-1} (Newton in long
+// double from Chebyshev-Lobatto guesses), w_j = 2/(p(p+1) L_p(x_j)^2).
+static void legendre_pair(int n, long double x, long double* L, long double* dL) {
+    long double a = 1, b = x, da = 0, db = 1;
+    if (n == 0) { *L = 1; *dL = 0; return; }
+    for (int k = 2; k <= n; k++) {
+        long double c = ((2 * k - 1) * x * b - (k - 1) * a) / k;
+        long double dc = da + (2 * k - 1) * b;
+        a = b; b = c; da = db; db = dc;
+    }
+    *L = b; *dL = db;
+}
+
+static void lgl_nodes(int p, double* x, double* w) {
+    static const double tx[3][4] = {{-1.0, 1.0}, {-1.0, 0.0, 1.0},
+                                    {-1.0, -0.4472135954999579, 0.4472135954999579, 1.0}};
+    static const double tw[3][4] = {{1.0, 1.0},
+                                    {0.33333333333333333, 1.33333333333333333, 0.33333333333333333},
+                                    {0.16666666666666667, 0.83333333333333333, 0.83333333333333333,
+                                     0.16666666666666667}};
+    if (p <= 3) {
+        for (int i = 0; i <= p; i++) { x[i] = tx[p - 1][i]; w[i] = tw[p - 1][i]; }
+        return;
+    }
+    long double xs[16];
+    xs[0] = -1; xs[p] = 1;
+    for (int j = 1; j < p; j++) {
+        long double t = -cosl(3.14159265358979323846264338327950288L * j / p);
+        for (int it = 0; it < 100; it++) {
+            long double A, dA, B, dB;
+            legendre_pair(p + 1, t, &A, &dA);
+            legendre_pair(p - 1, t, &B, &dB);
+            long double step = (A - B) / (dA - dB);
+            t -= step;
+            if (fabsl(step) < 1e-19L) break;
+        }
+        xs[j] = t;
+    }
+    for (int j = 0; j <= p; j++) {
+        long double L, dL;
+        legendre_pair(p, xs[j], &L, &dL);
+        x[j] = (double)xs[j];
+        w[j] = (double)(2.0L / ((long double)p * (p + 1) * L * L));
+    }
+    for (int j = 0; j <= p / 2; j++) {   // enforce the exact mirror symmetry
+        double a = 0.5 * (x[p - j] - x[j]), b = 0.5 * (w[j] + w[p - j]);
+        x[j] = -a; x[p - j] = a; w[j] = b; w[p - j] = b;
+    }
+    if (p % 2 == 0) x[p / 2] = 0.0;
+}
+
+// Barycentric D_ik = l_k'(x_i) with the negative-sum diagonal (:160-179).
+static void diff_matrix(int n, const double* x, double* D) {
+    double b[16];
+    for (int k = 0; k < n; k++) {
+        b[k] = 1.0;
+        for (int m = 0; m < n; m++) if (m != k) b[k] /= (x[k] - x[m]);
+    }
+    for (int i = 0; i < n; i++) {
+        double diag = 0.0;
+        for (int k = 0; k < n; k++) {
+            if (k == i) continue;
+            D[i * n + k] = (b[k] / b[i]) / (x[i] - x[k]);
+            diag -= D[i * n + k];
+        }
+        D[i * n + i] = diag;
+    }
+}
+
+// ── small kernels ──────────────────────────────────────────────────────────
+struct ICArgs {
+    int kind, K, N;
+    double x0, y0, L, h, gamma;
+    double rho_hi, p_hi, eps, xc, yc, u0, v0;
+    unsigned long long seed;
+    double amp;
+    const double* xi;
+};
+
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long z) {
+    // SplitMix64 finaliser (Steele, Lea, Flood 2014)
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// IC at the LGL nodes (ic_kernel :692-717; vortex q1_dg_euler_2d.cu:2166-2188;
+// KH step layer with counter-based noise keyed by (seed, node)).  Coordinates
+// and the branch tests are evaluated without FMA contraction so that a node on
+// an element face lands on the same side of a discontinuity as on the host.
+__global__ void ic_kernel(double* Q, ICArgs a) {
+#pragma clang fp contract(off)
+    const int N = a.N, N2 = N * N, NEL = a.K * a.K;
+    const size_t S = (size_t)NEL * N2;
+    const double g = a.gamma, pi = 3.14159265358979323846;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < S; c += (size_t)blockDim.x * gridDim.x) {
+        const int el = (int)(c / N2), nd = (int)(c - (size_t)el * N2);
+        const int ej = el / a.K, ei = el - ej * a.K, jn = nd / N, in = nd - jn * N;
+        const double x = a.x0 + a.h * ((double)ei + 0.5 * (a.xi[in] + 1.0));
+        const double y = a.y0 + a.h * ((double)ej + 0.5 * (a.xi[jn] + 1.0));
+        double r, u, v, p;
+        if (a.kind == DG_IC_CIRCULAR_SOD) {
+            const double cx = a.x0 + 0.5 * a.L, cy = a.y0 + 0.5 * a.L;
+            const double rr = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy));
+            const double phi = 0.5 * (1.0 + tanh((0.25 * a.L - rr) / (0.5 * a.h)));
+            r = 0.125 + (a.rho_hi - 0.125) * phi; u = 0.0; v = 0.0; p = 0.1 + (a.p_hi - 0.1) * phi;
+        } else if (a.kind == DG_IC_VORTEX) {
+            const double dx = x - a.xc, dy = y - a.yc, r2 = dx * dx + dy * dy;
+            const double f = a.eps / (2.0 * pi) * exp(0.5 * (1.0 - r2));
+            u = a.u0 - f * dy;
+            v = a.v0 + f * dx;
+            double T = 1.0 + (-(g - 1.0) * a.eps * a.eps / (8.0 * g * pi * pi) * exp(1.0 - r2));
+            if (T < 1e-6) T = 1e-6;
+            r = pow(T, 1.0 / (g - 1.0));
+            p = pow(T, g / (g - 1.0));
+        } else {
+            const unsigned long long z = splitmix64(a.seed + ((unsigned long long)c + 1ull) * 0x9e3779b97f4a7c15ull);
+            const double uni = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+            const bool inner = fabs(y) < 0.5;
+            r = inner ? 2.0 : 1.0;
+            u = inner ? 0.5 : -0.5;
+            v = 0.1 * sin(2.0 * pi * x) + a.amp * (2.0 * uni - 1.0);
+            p = 2.5;
+        }
+        Q[c] = r;
+        Q[S + c] = r * u;
+        Q[2 * S + c] = r * v;
+        Q[3 * S + c] = p / (g - 1.0) + 0.5 * r * (u * u + v * v);
+    }
+}
+
+// Unfused Shu-Osher stage + sanitize_cons (rk3_s1/s2/s3, :533-576).
+__global__ void rk3_kernel(double* Uo, const double* U0, const double* Up, const double* L, double dt, int stage,
+                           size_t S, double g) {
+    for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < S; n += (size_t)blockDim.x * gridDim.x) {
+        double Uc[4];
+        for (int q = 0; q < 4; q++) {
+            const size_t i = q * S + n;
+            if (stage == 1) Uc[q] = U0[i] + dt * L[i];
+            else if (stage == 2) Uc[q] = 0.75 * U0[i] + 0.25 * (Up[i] + dt * L[i]);
+            else Uc[q] = (1.0 / 3.0) * U0[i] + (2.0 / 3.0) * (Up[i] + dt * L[i]);
+        }
+        Cons64::sanitize_cons(Uc, g);
+        for (int q = 0; q < 4; q++) Uo[q * S + n] = Uc[q];
+    }
+}
+
+// Max wave speed of a state, for the first step of the fused stepping.
+__global__ void lambda_kernel(const double* Q, size_t S, double g, unsigned long long* dst) {
+    __shared__ double sred[BLOCK / 64];
+    double m = 0.0;
+    for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < S; n += (size_t)blockDim.x * gridDim.x) {
+        double W[4];
+        load_prim64(Q, S, n, g, W);
+        m = nanmax(m, Cons64::wave_speed(W, g));
+    }
+    block_max_atomic(m, dst, sred);
+}
+
+// max of an array (reduce_max/gpu_max, :646-659) into one bit-pattern slot
+__global__ void reduce_max_kernel(const double* v, int n, unsigned long long* dst) {
+    __shared__ double sred[BLOCK / 64];
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += blockDim.x * gridDim.x) m = nanmax(m, v[i]);
+    block_max_atomic(m, dst, sred);
+}
+
+__global__ void state_reset_kernel(DevState* st, double t0) {
+    st->lam_bits[0] = 0ull;
+    st->lam_bits[1] = 0ull;
+    st->t = t0;
+    st->dt = 0.0;
+    st->status = ST_OK;
+    st->skip = 0;
+}
+
+__global__ void slot_reset_kernel(unsigned long long* slot) { *slot = 0ull; }
+
+// GLL-weighted integrals of rho, rho u, rho v, E and -rho s/(g-1); one
+// partial per block, summed on the host in block order (deterministic).
+__global__ void totals_kernel(const double* Q, const double* w, int N, size_t S, double jw, double g,
+                              double* part) {
+    __shared__ double sred[5][BLOCK / 64];
+    double acc[5] = {0, 0, 0, 0, 0};
+    const int N2 = N * N;
+    for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < S; n += (size_t)blockDim.x * gridDim.x) {
+        const int nd = (int)(n % N2), jn = nd / N, in = nd - jn * N;
+        const double wij = jw * w[in] * w[jn];
+        double U[4] = {Q[n], Q[S + n], Q[2 * S + n], Q[3 * S + n]};
+        for (int q = 0; q < 4; q++) acc[q] += wij * U[q];
+        const double p = (g - 1.0) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0]);
+        const double s = log(p) - g * log(U[0]);
+        acc[4] += wij * (-U[0] * s / (g - 1.0));
+    }
+    for (int q = 0; q < 5; q++) {
+        double v = acc[q];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        acc[q] = v;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int q = 0; q < 5; q++) sred[q][wid] = acc[q];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 5; q++) {
+            double s = 0.0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += sred[q][i];
+            part[(size_t)blockIdx.x * 5 + q] = s;
+        }
+}
+
+// ── dispatch tables ───────────────────────────────────────────────────────
+static StageLauncher stage_launcher(int p) {
+    static const StageLauncher t[] = {nullptr, launch_stage_p1, launch_stage_p2, launch_stage_p3,
+                                      launch_stage_p4, launch_stage_p5, launch_stage_p6, launch_stage_p7};
+    return t[p];
+}
+static LimitLauncher limit_launcher(int p) {
+    static const LimitLauncher t[] = {nullptr, launch_limit_p1, launch_limit_p2, launch_limit_p3,
+                                      launch_limit_p4, launch_limit_p5, launch_limit_p6, launch_limit_p7};
+    return t[p];
+}
+static SensorLauncher sensor_launcher(int p) {
+    static const SensorLauncher t[] = {nullptr, launch_sensor_p1, launch_sensor_p2, launch_sensor_p3,
+                                       launch_sensor_p4, launch_sensor_p5, launch_sensor_p6, launch_sensor_p7};
+    return t[p];
+}
+
+static int grid_for(size_t n, int block) {
+    size_t g = (n + block - 1) / block;
+    return (int)std::min<size_t>(std::max<size_t>(g, 1), 8192);
+}
+
+static int max_elems_per_block(int N) { return BLOCK / (2 * N); }
+
+// Default schedule: the largest tile that still gives >= 2048 workgroups
+// (8 per CU) and at least one full wave of line threads.
+static int default_E(int N, int NEL) {
+    const int Emax = max_elems_per_block(N);
+    const int Emin = std::min(Emax, (64 + 2 * N - 1) / (2 * N));
+    int E = Emax;
+    while (E > Emin && (NEL + E - 1) / E < 2048) E = std::max(Emin, E / 2);
+    return E;
+}
+
+// ── stage launch ───────────────────────────────────────────────────────────
+struct StageCall {
+    int mode, stage;
+    const double *Up, *U0;
+    double* out;
+    double* lam_el;
+    double ca, cb, cfl, t_end;
+    int want_lambda, slot_cur, slot_next;
+};
+
+static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd) {
+    const dg_config& f = c->cfg;
+    const bool fp32 = f.precision == DG_MIXED;
+    const bool lam = s.mode == MODE_RHS && s.lam_el != nullptr;
+    const bool lim = s.mode == MODE_STAGE && f.limiter;
+    const bool full = f.av || lim || lam;
+    LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, lim, f.av != 0, lam);
+    KArgs a;
+    a.Up = s.Up; a.U0 = s.U0; a.out = s.out; a.lam_el = s.lam_el;
+    a.tab = fp32 ? c->d_tab32 : c->d_tab64;
+    a.w64 = (const double*)c->d_tab64 + c->N2;
+    a.st = c->d_st;
+    a.K = f.K; a.NEL = c->NEL; a.E = E; a.xcd = xcd;
+    a.mode = s.mode; a.stage = s.stage; a.vol = f.volume_flux; a.surf = f.surface_flux; a.bc = f.bc;
+    a.av = f.av; a.limiter = f.limiter; a.want_lambda = s.want_lambda;
+    a.slot_cur = s.slot_cur; a.slot_next = s.slot_next;
+    a.h = c->h; a.gamma = f.gamma; a.ca = s.ca; a.cb = s.cb; a.cfl = s.cfl; a.t_end = s.t_end;
+    a.av_cav = f.av_cav; a.av_k = f.av_ksensor; a.eps_rho = f.eps_rho; a.eps_p = f.eps_p;
+    a.off_U = L.off_U; a.off_av = L.off_av; a.off_lam = L.off_lam; a.off_el = L.off_el;
+    const int grid = (c->NEL + E - 1) / E;
+    const int block = std::min(BLOCK, (E * 2 * c->N + 63) / 64 * 64);
+    HIPCK(c, stage_launcher(f.p)(a, fp32, full, grid, block, (size_t)L.total, c->stream));
+    return DG_OK;
+}
+
+// ── exported API ───────────────────────────────────────────────────────────
+extern "C" {
+
+void dg_config_default(dg_config* f) {
+    if (!f) return;
+    memset(f, 0, sizeof(*f));
+    f->K = 50; f->p = 3; f->x0 = 0.0; f->y0 = 0.0; f->L = 1.0; f->gamma = 1.4;
+    f->volume_flux = DG_VOL_CHANDRASHEKAR; f->surface_flux = DG_SURF_HLLC; f->bc = DG_BC_WALL;
+    f->av = 1; f->limiter = 1; f->precision = DG_FP64;
+    f->av_cav = 0.25; f->av_ksensor = 0.1; f->eps_rho = 1e-6; f->eps_p = 1e-6;
+    f->elems_per_block = 0; f->xcd_remap = -1;
+}
+
+void dg_ic_default(dg_ic_params* ic, int kind) {
+    if (!ic) return;
+    memset(ic, 0, sizeof(*ic));
+    ic->kind = kind;
+    ic->rho_hi = 11.0; ic->p_hi = 10.0;
+    ic->vortex_eps = 5.0; ic->xc = 5.0; ic->yc = 5.0; ic->u0 = 1.0; ic->v0 = 1.0;
+    ic->seed = 1234; ic->noise_amp = 1e-3;
+}
+
+const char* dg_status_string(int s) {
+    switch (s) {
+        case DG_OK: return "ok";
+        case DG_ERR_INVALID_ARG: return "invalid argument";
+        case DG_ERR_HIP: return "HIP runtime error";
+        case DG_ERR_NONFINITE: return "non-finite or non-positive max wave speed";
+        case DG_ERR_DT_UNDERFLOW: return "dt underflow";
+        case DG_ERR_BLOWUP: return "blow-up (max wave speed > 1e6)";
+        default: return "unknown status";
+    }
+}
+
+const char* dg_last_error(const dg_ctx* c) { return c ? c->err : "null context"; }
+
+long long dg_state_size(const dg_ctx* c) { return c ? 4LL * c->NEL * c->N2 : 0; }
+long long dg_num_elements(const dg_ctx* c) { return c ? c->NEL : 0; }
+
+int dg_operators(const dg_ctx* c, double* x, double* w, double* D) {
+    if (!c) return DG_ERR_INVALID_ARG;
+    for (int i = 0; i < c->N; i++) {
+        if (x) x[i] = c->x[i];
+        if (w) w[i] = c->w[i];
+    }
+    if (D) memcpy(D, c->D, sizeof(double) * c->N2);
+    return DG_OK;
+}
+
+int dg_destroy(dg_ctx* c) {
+    if (!c) return DG_OK;
+    hipFree(c->d_tab64); hipFree(c->d_tab32); hipFree(c->d_U1); hipFree(c->d_U2);
+    hipFree(c->d_st); hipFree(c->d_red); hipFree(c->d_part); hipFree(c->d_xi);
+    delete c;
+    return DG_OK;
+}
+
+int dg_create(const dg_config* cfg, void* stream, dg_ctx** out) {
+    if (!cfg || !out) return DG_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (cfg->K < 1 || cfg->p < 1 || cfg->p > PMAX_GPU || !(cfg->L > 0) || !(cfg->gamma > 1.0) ||
+        cfg->volume_flux < 0 || cfg->volume_flux > 2 || cfg->surface_flux < 0 || cfg->surface_flux > 1 ||
+        cfg->bc < 0 || cfg->bc > 1 || cfg->precision < 0 || cfg->precision > 1 ||
+        (long long)cfg->K * cfg->K > (1LL << 30) / ((cfg->p + 1) * (cfg->p + 1)))
+        return DG_ERR_INVALID_ARG;
+    dg_ctx* c = new dg_ctx();
+    c->cfg = *cfg;
+    c->stream = (hipStream_t)stream;
+    c->N = cfg->p + 1; c->N2 = c->N * c->N; c->NEL = cfg->K * cfg->K;
+    c->h = cfg->L / cfg->K;
+    lgl_nodes(cfg->p, c->x, c->w);
+    diff_matrix(c->N, c->x, c->D);
+
+    const int Emax = max_elems_per_block(c->N);
+    c->E = cfg->elems_per_block > 0 ? std::min(cfg->elems_per_block, Emax) : default_E(c->N, c->NEL);
+    c->xcd = cfg->xcd_remap >= 0 ? (cfg->xcd_remap != 0) : 1;
+
+    // tables: D[N2], w[N], winv[N]
+    std::vector<double> t64(c->N2 + 2 * c->N);
+    std::vector<float> t32(c->N2 + 2 * c->N);
+    for (int i = 0; i < c->N2; i++) t64[i] = c->D[i];
+    for (int i = 0; i < c->N; i++) { t64[c->N2 + i] = c->w[i]; t64[c->N2 + c->N + i] = 1.0 / c->w[i]; }
+    for (size_t i = 0; i < t64.size(); i++) t32[i] = (float)t64[i];
+
+    const size_t S = (size_t)c->NEL * c->N2;
+    c->part_blocks = grid_for(S, BLOCK);
+#define CK_CREATE(x)                                                                           \
+    do {                                                                                       \
+        hipError_t _e = (x);                                                                   \
+        if (_e != hipSuccess) { dg_destroy(c); return DG_ERR_HIP; }                           \
+    } while (0)
+    CK_CREATE(hipMalloc(&c->d_tab64, t64.size() * sizeof(double)));
+    CK_CREATE(hipMalloc(&c->d_tab32, t32.size() * sizeof(float)));
+    CK_CREATE(hipMalloc(&c->d_U1, 4 * S * sizeof(double)));
+    CK_CREATE(hipMalloc(&c->d_U2, 4 * S * sizeof(double)));
+    CK_CREATE(hipMalloc(&c->d_st, sizeof(DevState)));
+    CK_CREATE(hipMalloc(&c->d_red, sizeof(unsigned long long)));
+    CK_CREATE(hipMalloc(&c->d_part, (size_t)c->part_blocks * 5 * sizeof(double)));
+    CK_CREATE(hipMalloc(&c->d_xi, 16 * sizeof(double)));
+    CK_CREATE(hipMemcpyAsync(c->d_xi, c->x, c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CK_CREATE(hipMemcpyAsync(c->d_tab64, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CK_CREATE(hipMemcpyAsync(c->d_tab32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    CK_CREATE(hipStreamSynchronize(c->stream));
+#undef CK_CREATE
+    *out = c;
+    return DG_OK;
+}
+
+int dg_init_state(dg_ctx* c, const dg_ic_params* ic, double* Q) {
+    if (!c || !ic || !Q) return fail(c, DG_ERR_INVALID_ARG, "dg_init_state: null argument");
+    if (ic->kind < 0 || ic->kind > 2) return fail(c, DG_ERR_INVALID_ARG, "dg_init_state: unknown IC %d", ic->kind);
+    ICArgs a;
+    a.kind = ic->kind; a.K = c->cfg.K; a.N = c->N;
+    a.x0 = c->cfg.x0; a.y0 = c->cfg.y0; a.L = c->cfg.L; a.h = c->h; a.gamma = c->cfg.gamma;
+    a.rho_hi = ic->rho_hi; a.p_hi = ic->p_hi; a.eps = ic->vortex_eps; a.xc = ic->xc; a.yc = ic->yc;
+    a.u0 = ic->u0; a.v0 = ic->v0; a.seed = ic->seed; a.amp = ic->noise_amp;
+    a.xi = c->d_xi;
+    const size_t S = (size_t)c->NEL * c->N2;
+    hipLaunchKernelGGL(ic_kernel, dim3(grid_for(S, BLOCK)), dim3(BLOCK), 0, c->stream, Q, a);
+    HIPCK(c, hipGetLastError());
+    if (c->cfg.limiter) return dg_limit(c, Q);
+    return DG_OK;
+}
+
+int dg_limit(dg_ctx* c, double* Q) {
+    if (!c || !Q) return fail(c, DG_ERR_INVALID_ARG, "dg_limit: null argument");
+    const int E = std::max(1, max_elems_per_block(c->N) / 2);
+    HIPCK(c, limit_launcher(c->cfg.p)(Q, (const double*)c->d_tab64 + c->N2, c->NEL, E, c->cfg.gamma,
+                                       c->cfg.eps_rho, c->cfg.eps_p, c->stream));
+    return DG_OK;
+}
+
+int dg_compute_rhs(dg_ctx* c, const double* Q, double* RHS, double* lam_el) {
+    if (!c || !Q || !RHS) return fail(c, DG_ERR_INVALID_ARG, "dg_compute_rhs: null argument");
+    StageCall s{};
+    s.mode = MODE_RHS; s.stage = 0; s.Up = Q; s.U0 = Q; s.out = RHS; s.lam_el = lam_el;
+    return launch_stage(c, s, c->E, c->xcd);
+}
+
+int dg_rk3_stage(dg_ctx* c, int stage, double* Uout, const double* U0, const double* Uprev, const double* L,
+                 double dt) {
+    if (!c || !Uout || !U0 || !L || (stage != 1 && !Uprev) || stage < 1 || stage > 3)
+        return fail(c, DG_ERR_INVALID_ARG, "dg_rk3_stage: bad argument");
+    const size_t S = (size_t)c->NEL * c->N2;
+    hipLaunchKernelGGL(rk3_kernel, dim3(grid_for(S, BLOCK)), dim3(BLOCK), 0, c->stream, Uout, U0,
+                       stage == 1 ? U0 : Uprev, L, dt, stage, S, c->cfg.gamma);
+    HIPCK(c, hipGetLastError());
+    if (c->cfg.limiter) return dg_limit(c, Uout);
+    return DG_OK;
+}
+
+int dg_cfl_dt(dg_ctx* c, const double* lam_el, double cfl, double* lam_max, double* dt) {
+    if (!c || !lam_el) return fail(c, DG_ERR_INVALID_ARG, "dg_cfl_dt: null argument");
+    hipLaunchKernelGGL(slot_reset_kernel, dim3(1), dim3(1), 0, c->stream, c->d_red);
+    hipLaunchKernelGGL(reduce_max_kernel, dim3(grid_for(c->NEL, BLOCK)), dim3(BLOCK), 0, c->stream, lam_el,
+                       c->NEL, c->d_red);
+    HIPCK(c, hipGetLastError());
+    unsigned long long bits = 0;
+    HIPCK(c, hipMemcpyAsync(&bits, c->d_red, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    double m;
+    memcpy(&m, &bits, sizeof(m));
+    if (lam_max) *lam_max = m;
+    if (!(m > 0.0)) return fail(c, DG_ERR_NONFINITE, "max wave speed %g", m);
+    if (dt) *dt = cfl * c->h / (m * c->N);
+    return DG_OK;
+}
+
+static int advance_impl(dg_ctx* c, double* Q, int nsteps, double cfl, double t_end, double* t_inout,
+                        double* dt_last, int E, int xcd) {
+    const size_t S = (size_t)c->NEL * c->N2;
+    const double t0 = t_inout ? *t_inout : 0.0;
+    hipLaunchKernelGGL(state_reset_kernel, dim3(1), dim3(1), 0, c->stream, c->d_st, t0);
+    hipLaunchKernelGGL(lambda_kernel, dim3(grid_for(S, BLOCK)), dim3(BLOCK), 0, c->stream, Q, S, c->cfg.gamma,
+                       &c->d_st->lam_bits[0]);
+    HIPCK(c, hipGetLastError());
+    for (int n = 0; n < nsteps; n++) {
+        const int cur = n & 1, nxt = cur ^ 1;
+        StageCall s{};
+        s.mode = MODE_STAGE; s.cfl = cfl; s.t_end = t_end; s.slot_cur = cur; s.slot_next = nxt;
+        s.stage = 1; s.Up = Q; s.U0 = Q; s.out = c->d_U1; s.ca = 0.0; s.cb = 1.0; s.want_lambda = 0;
+        int rc = launch_stage(c, s, E, xcd);
+        if (rc) return rc;
+        s.stage = 2; s.Up = c->d_U1; s.U0 = Q; s.out = c->d_U2; s.ca = 0.75; s.cb = 0.25;
+        if ((rc = launch_stage(c, s, E, xcd))) return rc;
+        s.stage = 3; s.Up = c->d_U2; s.U0 = Q; s.out = Q; s.ca = 1.0 / 3.0; s.cb = 2.0 / 3.0; s.want_lambda = 1;
+        if ((rc = launch_stage(c, s, E, xcd))) return rc;
+    }
+    DevState h;
+    HIPCK(c, hipMemcpyAsync(&h, c->d_st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if (t_inout) *t_inout = h.t;
+    if (dt_last) *dt_last = h.dt;
+    if (h.status != ST_OK) return fail(c, h.status, "%s at t=%.9g", dg_status_string(h.status), h.t);
+    return DG_OK;
+}
+
+int dg_advance(dg_ctx* c, double* Q, int nsteps, double cfl, double t_end, double* t_inout, double* dt_last) {
+    if (!c || !Q || nsteps < 0 || !(cfl > 0)) return fail(c, DG_ERR_INVALID_ARG, "dg_advance: bad argument");
+    return advance_impl(c, Q, nsteps, cfl, t_end, t_inout, dt_last, c->E, c->xcd);
+}
+
+int dg_step(dg_ctx* c, double* Q, double cfl, double* dt_used) {
+    return dg_advance(c, Q, 1, cfl, 0.0, nullptr, dt_used);
+}
+
+int dg_sensor(dg_ctx* c, const double* Q, double* sensor_el) {
+    if (!c || !Q || !sensor_el) return fail(c, DG_ERR_INVALID_ARG, "dg_sensor: null argument");
+    HIPCK(c, sensor_launcher(c->cfg.p)(Q, sensor_el, (const double*)c->d_tab64, c->cfg.K, c->h, c->cfg.gamma,
+                                        c->cfg.av_ksensor, c->stream));
+    return DG_OK;
+}
+
+int dg_totals(dg_ctx* c, const double* Q, double out[5]) {
+    if (!c || !Q || !out) return fail(c, DG_ERR_INVALID_ARG, "dg_totals: null argument");
+    const size_t S = (size_t)c->NEL * c->N2;
+    hipLaunchKernelGGL(totals_kernel, dim3(c->part_blocks), dim3(BLOCK), 0, c->stream, Q,
+                       (const double*)c->d_tab64 + c->N2, c->N, S, 0.25 * c->h * c->h, c->cfg.gamma, c->d_part);
+    HIPCK(c, hipGetLastError());
+    std::vector<double> part((size_t)c->part_blocks * 5);
+    HIPCK(c, hipMemcpyAsync(part.data(), c->d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 5; q++) {
+        double s = 0.0;
+        for (int b = 0; b < c->part_blocks; b++) s += part[(size_t)b * 5 + q];
+        out[q] = s;
+    }
+    return DG_OK;
+}
+
+// Schedule tuner: run a few fused steps on the ctx's own stage buffers for
+// each candidate (elements per block x XCD remap) and keep the fastest.
+int dg_autotune(dg_ctx* c, int* elems_per_block, int* xcd_remap) {
+    if (!c) return DG_ERR_INVALID_ARG;
+    const size_t S = (size_t)c->NEL * c->N2;
+    double* Q = nullptr;
+    HIPCK(c, hipMalloc(&Q, 4 * S * sizeof(double)));
+    dg_ic_params ic;
+    dg_ic_default(&ic, DG_IC_VORTEX);
+    ic.xc = c->cfg.x0 + 0.5 * c->cfg.L; ic.yc = c->cfg.y0 + 0.5 * c->cfg.L;
+    int rc = dg_init_state(c, &ic, Q);
+    const int Emax = max_elems_per_block(c->N);
+    const int Emin = std::max(1, (64 + 2 * c->N - 1) / (2 * c->N) / 2);
+    std::vector<int> Es;
+    for (int E = Emax; E >= Emin; E /= 2) Es.push_back(E);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    float best = 1e30f;
+    int bestE = c->E, bestX = c->xcd;
+    for (int x = 0; x < 2 && rc == DG_OK; x++)
+        for (int E : Es) {
+            double t = 0.0;
+            rc = advance_impl(c, Q, 1, 0.5, 0.0, &t, nullptr, E, x);   // warm-up (LDS attribute, caches)
+            if (rc) break;
+            hipEventRecord(e0, c->stream);
+            rc = advance_impl(c, Q, 3, 0.5, 0.0, &t, nullptr, E, x);
+            hipEventRecord(e1, c->stream);
+            hipEventSynchronize(e1);
+            if (rc) break;
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) { best = ms; bestE = E; bestX = x; }
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(Q);
+    if (rc) return rc;
+    c->E = bestE;
+    c->xcd = bestX;
+    if (elems_per_block) *elems_per_block = bestE;
+    if (xcd_remap) *xcd_remap = bestX;
+    return DG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,224 @@
+// Pointwise Euler physics on the device, templated on the compute type.
+//
+// Follows dgsem_lobatto_2d.cu:184-355 (sound speed, prim/cons conversion with
+// the 1e-14 floors, the physical flux, the logarithmic mean, the Chandrashekar
+// EC flux, HLLC, the wall ghost state) plus two closed-form additions that the
+// reference does not contain: the Ranocha EC/KEP flux and the Rusanov flux.
+//
+// Real = double: every operation is IEEE FP64 (divides and logs are the
+// compiler's correctly rounded / OCML sequences).
+// Real = float : the mixed form.  Reciprocal, log2 and sqrt use the gfx950
+// transcendental unit (v_rcp_f32, v_log_f32, v_sqrt_f32, about 1 ulp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dg {
+
+template <typename Real> struct M;
+
+template <> struct M<double> {
+    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ double ln(double x) { return log(x); }
+    static __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+    static __device__ __forceinline__ double fmax_(double a, double b) { return fmax(a, b); }
+    static __device__ __forceinline__ double fmin_(double a, double b) { return fmin(a, b); }
+    static __device__ __forceinline__ double abs_(double a) { return fabs(a); }
+    // below this squared ratio (a-b)/(a+b) the log mean uses its series
+    static constexpr double series_thr = 1e-4;
+};
+
+template <> struct M<float> {
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float ln(float x) {
+        return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
+    }
+    static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float fmax_(float a, float b) { return fmaxf(a, b); }
+    static __device__ __forceinline__ float fmin_(float a, float b) { return fminf(a, b); }
+    static __device__ __forceinline__ float abs_(float a) { return fabsf(a); }
+    // the 4-term series is accurate to 3e-10 at u = 1e-2, far below FP32 epsilon
+    static constexpr float series_thr = 1e-2f;
+};
+
+// Primitive node state held in registers / LDS: rho, u, v, p and z = rho/p.
+template <typename Real> struct Prim {
+    Real r, u, v, p, z;
+};
+
+// ── FP64 state-side helpers (sanitize + prim conversion), :208-229, :347-355 ──
+struct Cons64 {
+    static __device__ __forceinline__ double pressure_from_cons(const double U[4], double g) {
+        double rho = fmax(U[0], 1e-14);
+        double ke = 0.5 * (U[1] * U[1] + U[2] * U[2]) / rho;
+        return (g - 1.0) * (U[3] - ke);
+    }
+    static __device__ __forceinline__ void sanitize_cons(double U[4], double g) {
+        U[0] = fmax(U[0], 1e-14);
+        double p = pressure_from_cons(U, g);
+        if (p < 1e-14) U[3] = 1e-14 / (g - 1.0) + 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0];
+    }
+    // cons_to_prim_sane: sanitize, convert, floor rho/p, clamp |u|,|v| <= 1e4
+    static __device__ __forceinline__ void to_prim(const double Uin[4], double W[4], double g) {
+        double U[4] = {Uin[0], Uin[1], Uin[2], Uin[3]};
+        sanitize_cons(U, g);
+        W[0] = U[0];
+        W[1] = U[1] / U[0];
+        W[2] = U[2] / U[0];
+        W[3] = (g - 1.0) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0]);
+        W[0] = fmax(W[0], 1e-14);
+        W[3] = fmax(W[3], 1e-14);
+        if (W[0] < 1e-5) { W[1] = 0.0; W[2] = 0.0; return; }
+        W[1] = fmax(fmin(W[1], 1e4), -1e4);
+        W[2] = fmax(fmin(W[2], 1e4), -1e4);
+    }
+    // max wave speed |u|+|v|+c of sanitized primitives (:483-486)
+    static __device__ __forceinline__ double wave_speed(const double W[4], double g) {
+        double rho = fmax(W[0], 1e-12), c2 = g * W[3] / rho;
+        return fabs(W[1]) + fabs(W[2]) + sqrt(fmax(c2, 1e-14));
+    }
+};
+
+template <typename Real>
+__device__ __forceinline__ Prim<Real> make_prim(const double W[4]) {
+    Prim<Real> w;
+    w.r = (Real)W[0]; w.u = (Real)W[1]; w.v = (Real)W[2]; w.p = (Real)W[3];
+    w.z = (Real)(W[0] / W[3]);
+    return w;
+}
+
+// ── compute-precision physics ─────────────────────────────────────────────
+template <typename Real> struct Phys {
+    using Mx = M<Real>;
+    Real g, gm1, inv_gm1;
+
+    __device__ __forceinline__ Real sound_speed(Real p, Real rho) const {   // :184-188
+        Real cs2 = Mx::div(g * p, rho);
+        return Mx::sqrt_(cs2 > (Real)1e-14 ? cs2 : (Real)1e-14);
+    }
+    // physical flux along x (dir 0) or y (dir 1), :232-241
+    template <int DIR>
+    __device__ __forceinline__ void flux(const Prim<Real>& w, Real F[4]) const {
+        Real un = DIR == 0 ? w.u : w.v;
+        Real E = w.p * inv_gm1 + (Real)0.5 * w.r * (w.u * w.u + w.v * w.v);
+        F[0] = w.r * un;
+        F[1] = w.r * un * w.u + (DIR == 0 ? w.p : (Real)0);
+        F[2] = w.r * un * w.v + (DIR == 1 ? w.p : (Real)0);
+        F[3] = (E + w.p) * un;
+    }
+    __device__ __forceinline__ void flux_dir(int dir, const Prim<Real>& w, Real F[4]) const {
+        if (dir == 0) flux<0>(w, F); else flux<1>(w, F);
+    }
+
+    // Log mean L(a,b) = (a-b)/ln(a/b) and its inverse.  With f = (a-b)/(a+b),
+    // ln(a/b) = 2 atanh(f), so L = (a+b)/2 * f/atanh(f); for small f the
+    // series atanh(f)/f = 1 + f^2/3 + f^4/5 + f^6/7 (the reference's form,
+    // :244-253) and its reciprocal 1 - f^2/3 - 4f^4/45 - 44f^6/945 are used.
+    // The log branch keeps the ratio form ln(a/b), well conditioned in FP32.
+    // Log means of one pair: the series form first (no log, one reciprocal
+    // each), replaced by the ratio-form log only where the series does not
+    // apply.  That branch is skipped by the whole wave on smooth data.
+    __device__ __forceinline__ static void pair_means(Real aL, Real aR, Real bL, Real bR, Real& a_ln,
+                                                      Real& inv_b_ln) {
+        Real sa = aL + aR, sb = bL + bR;
+        Real ra = Mx::rcp(sa), rb = Mx::rcp(sb);
+        Real fa = (aL - aR) * ra, ua = fa * fa;
+        Real fb = (bL - bR) * rb, ub = fb * fb;
+        a_ln = (Real)0.5 * sa *
+               ((Real)1 - ua * ((Real)(1.0 / 3.0) + ua * ((Real)(4.0 / 45.0) + ua * (Real)(44.0 / 945.0))));
+        inv_b_ln = (Real)2 * rb *
+               ((Real)1 + ub * ((Real)(1.0 / 3.0) + ub * ((Real)(1.0 / 5.0) + ub * (Real)(1.0 / 7.0))));
+        if (ua >= (Real)Mx::series_thr) a_ln = Mx::div(aL - aR, Mx::ln(Mx::div(aL, aR)));
+        if (ub >= (Real)Mx::series_thr) inv_b_ln = Mx::div(Mx::ln(Mx::div(bL, bR)), bL - bR);
+    }
+
+    // Ranocha EC + KEP two-point flux along x (y-lines pass u,v swapped).
+    __device__ __forceinline__ void ranocha(const Prim<Real>& L, const Prim<Real>& R, Real F[4]) const {
+        Real r_ln, inv_z_ln;
+        pair_means(L.r, R.r, L.z, R.z, r_ln, inv_z_ln);
+        Real u_av = (Real)0.5 * (L.u + R.u), v_av = (Real)0.5 * (L.v + R.v);
+        Real p_av = (Real)0.5 * (L.p + R.p);
+        Real eint = (Real)0.5 * (L.u * R.u + L.v * R.v) + inv_gm1 * inv_z_ln;
+        Real f1 = r_ln * u_av;
+        F[0] = f1; F[1] = f1 * u_av + p_av; F[2] = f1 * v_av;
+        F[3] = f1 * eint + (Real)0.5 * (L.p * R.u + R.p * L.u);
+    }
+
+    // Chandrashekar EC flux (:259-285) along x, beta = rho/(2p) = z/2.
+    __device__ __forceinline__ void chandrashekar(const Prim<Real>& L, const Prim<Real>& R, Real F[4]) const {
+        Real bL = (Real)0.5 * L.z, bR = (Real)0.5 * R.z;
+        Real r_ln, inv_b_ln;
+        pair_means(L.r, R.r, bL, bR, r_ln, inv_b_ln);
+        // p_hat = rho_avg/(2 beta_avg) = (rL+rR)/(2 (bL+bR))
+        Real p_hat = Mx::div((Real)0.5 * (L.r + R.r), bL + bR);
+        Real u_av = (Real)0.5 * (L.u + R.u), v_av = (Real)0.5 * (L.v + R.v);
+        Real vel2 = (Real)0.5 * (L.u * L.u + L.v * L.v) + (Real)0.5 * (R.u * R.u + R.v * R.v);
+        Real e_int = (Real)0.5 * (inv_gm1 * inv_b_ln - vel2);
+        Real f1 = r_ln * u_av, f2 = f1 * u_av + p_hat, f3 = f1 * v_av;
+        F[0] = f1; F[1] = f2; F[2] = f3; F[3] = f1 * e_int + f2 * u_av + f3 * v_av;
+    }
+
+    // HLLC in the rotated frame, :288-335.  Normal (1,0) for DIR 0, (0,1) for DIR 1.
+    __device__ __forceinline__ void hllc(int dir, const Prim<Real>& L, const Prim<Real>& R, Real F[4]) const {
+        Real nx = dir == 0 ? (Real)1 : (Real)0, ny = dir == 0 ? (Real)0 : (Real)1;
+        Real unL = L.u * nx + L.v * ny, unR = R.u * nx + R.v * ny;
+        Real utL = -L.u * ny + L.v * nx, utR = -R.u * ny + R.v * nx;
+        Real cL = sound_speed(L.p, L.r), cR = sound_speed(R.p, R.r);
+        Real EL = L.p * inv_gm1 + (Real)0.5 * L.r * (L.u * L.u + L.v * L.v);
+        Real ER = R.p * inv_gm1 + (Real)0.5 * R.r * (R.u * R.u + R.v * R.v);
+        Real SL = Mx::fmin_(unL - cL, unR - cR);
+        Real SR = Mx::fmax_(unL + cL, unR + cR);
+        if (SL >= (Real)0) {
+            F[0] = L.r * unL; F[1] = L.r * unL * L.u + L.p * nx; F[2] = L.r * unL * L.v + L.p * ny;
+            F[3] = (EL + L.p) * unL;
+        } else if (SR <= (Real)0) {
+            F[0] = R.r * unR; F[1] = R.r * unR * R.u + R.p * nx; F[2] = R.r * unR * R.v + R.p * ny;
+            F[3] = (ER + R.p) * unR;
+        } else {
+            Real dL = L.r * (SL - unL), dR = R.r * (SR - unR);
+            Real Ss = Mx::div(R.p - L.p + dL * unL - dR * unR, dL - dR);
+            bool lft = Ss >= (Real)0;
+            Real rK = lft ? L.r : R.r, unK = lft ? unL : unR, utK = lft ? utL : utR;
+            Real pK = lft ? L.p : R.p, EK = lft ? EL : ER, SK = lft ? SL : SR;
+            Real fact = Mx::div(rK * (SK - unK), SK - Ss);
+            Real Us0 = fact, Us1 = fact * Ss, Us2 = fact * utK;
+            Real Us3 = fact * (Mx::div(EK, rK) + (Ss - unK) * (Ss + Mx::div(pK, rK * (SK - unK))));
+            Real Fl0 = rK * unK + SK * (Us0 - rK);
+            Real Fl1 = rK * unK * unK + pK + SK * (Us1 - rK * unK);
+            Real Fl2 = rK * unK * utK + SK * (Us2 - rK * utK);
+            Real Fl3 = (EK + pK) * unK + SK * (Us3 - EK);
+            F[0] = Fl0;
+            F[1] = Fl1 * nx - Fl2 * ny;
+            F[2] = Fl1 * ny + Fl2 * nx;
+            F[3] = Fl3;
+        }
+    }
+
+    // Rusanov: 1/2 (fL + fR) - 1/2 lam (UR - UL), lam = max(|un| + c).
+    __device__ __forceinline__ void rusanov(int dir, const Prim<Real>& L, const Prim<Real>& R, Real F[4]) const {
+        Real FL[4], FR[4];
+        flux_dir(dir, L, FL);
+        flux_dir(dir, R, FR);
+        Real unL = dir == 0 ? L.u : L.v, unR = dir == 0 ? R.u : R.v;
+        Real lam = Mx::fmax_(Mx::abs_(unL) + sound_speed(L.p, L.r), Mx::abs_(unR) + sound_speed(R.p, R.r));
+        Real EL = L.p * inv_gm1 + (Real)0.5 * L.r * (L.u * L.u + L.v * L.v);
+        Real ER = R.p * inv_gm1 + (Real)0.5 * R.r * (R.u * R.u + R.v * R.v);
+        Real hl = (Real)0.5 * lam;
+        F[0] = (Real)0.5 * (FL[0] + FR[0]) - hl * (R.r - L.r);
+        F[1] = (Real)0.5 * (FL[1] + FR[1]) - hl * (R.r * R.u - L.r * L.u);
+        F[2] = (Real)0.5 * (FL[2] + FR[2]) - hl * (R.r * R.v - L.r * L.v);
+        F[3] = (Real)0.5 * (FL[3] + FR[3]) - hl * (ER - EL);
+    }
+};
+
+// wall_state, :339-344: reflect the normal velocity
+template <typename Real>
+__device__ __forceinline__ Prim<Real> wall_ghost(const Prim<Real>& w, int dir) {
+    Prim<Real> g = w;
+    if (dir == 0) g.u = -w.u; else g.v = -w.v;
+    return g;
+}
+
+}  // namespace dg

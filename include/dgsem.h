@@ -1,0 +1,173 @@
+/*
+ * dgsem.h — C ABI of the MI355X-native 2-D compressible-Euler DGSEM step.
+ *
+ * The reference (KLozes/fvStuff, dgsem_lobatto_2d.cu) has no library interface:
+ * its "API" is the set of __global__ kernels plus the __constant__ operator
+ * tables that main() drives.  Each entry point below replaces one of them
+ * (file:line cited per function).  Everything here is plain C: device buffers
+ * are passed as raw pointers (hipMalloc'd by the caller), sizes as ints.
+ *
+ * State layout (identical to the reference's SIDX macro, dgsem_lobatto_2d.cu:126):
+ *   Q[var*NEL*NN2 + el*NN2 + jn*NN + in],  var in {rho, rho*u, rho*v, E},
+ *   el = ej*K + ei,  NN = p+1,  NN2 = NN*NN,  NEL = K*K.
+ * The state is always FP64.  DG_MIXED computes the right-hand side in FP32 and
+ * keeps the stage accumulation, reductions and positivity checks in FP64.
+ *
+ * Errors: every function returns DG_OK (0) or a positive status code; nothing
+ * calls exit() (the reference's CK macro exits, dgsem_lobatto_2d.cu:129-136).
+ * dg_last_error() gives a human-readable message for the last failure on a ctx.
+ */
+#ifndef DGSEM_H
+#define DGSEM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ── status codes ───────────────────────────────────────────────────────── */
+#define DG_OK                 0
+#define DG_ERR_INVALID_ARG    1  /* bad config / null pointer / unsupported p      */
+#define DG_ERR_HIP            2  /* a HIP runtime call failed                      */
+#define DG_ERR_NONFINITE      3  /* max wave speed not > 0 (NaN state), ref :1028  */
+#define DG_ERR_DT_UNDERFLOW   4  /* dt < 1e-14, ref :1032-1035                     */
+#define DG_ERR_BLOWUP         5  /* max wave speed > 1e6, ref :1054-1057           */
+
+/* ── switches ───────────────────────────────────────────────────────────── */
+enum dg_volume_flux {
+    DG_VOL_CHANDRASHEKAR = 0,  /* EC_FLUX=1 of the reference, :259-285, :417-430 */
+    DG_VOL_RANOCHA       = 1,  /* entropy-conservative + kinetic-energy-preserving */
+    DG_VOL_COLLOCATION   = 2   /* EC_FLUX=0 of the reference, :431-439           */
+};
+enum dg_surface_flux {
+    DG_SURF_HLLC    = 0,       /* hllc_n, :288-335                               */
+    DG_SURF_RUSANOV = 1        /* local Lax-Friedrichs                           */
+};
+enum dg_bc {
+    DG_BC_WALL     = 0,        /* wall_state reflecting ghost, :339-344          */
+    DG_BC_PERIODIC = 1
+};
+enum dg_precision {
+    DG_FP64  = 0,
+    DG_MIXED = 1               /* FP32 volume/surface, FP64 state and reductions */
+};
+enum dg_ic_kind {
+    DG_IC_CIRCULAR_SOD      = 0,  /* ic_kernel, :692-717                          */
+    DG_IC_VORTEX            = 1,  /* isentropic vortex, q1_dg_euler_2d.cu:2166-2188 */
+    DG_IC_KELVIN_HELMHOLTZ  = 2   /* step shear layer + seeded noise              */
+};
+
+/* Solver configuration.  dg_config_default() fills the reference's defaults
+ * (p=3, walls, Chandrashekar + HLLC, AV on, limiter on, [0,1]^2, FP64). */
+typedef struct dg_config {
+    int    K;             /* elements per side (NEL = K*K)                       */
+    int    p;             /* polynomial degree, 1..7                             */
+    double x0, y0;        /* lower-left corner of the square domain              */
+    double L;             /* side length; h = L/K (reference: L=1, :922)         */
+    double gamma;         /* ratio of specific heats (reference GAMMA_V=1.4)     */
+    int    volume_flux;   /* enum dg_volume_flux                                 */
+    int    surface_flux;  /* enum dg_surface_flux                                */
+    int    bc;            /* enum dg_bc                                          */
+    int    av;            /* Ducros-sensor artificial viscosity on/off (AV_ON)   */
+    int    limiter;       /* Zhang-Shu positivity limiter after each stage       */
+    int    precision;     /* enum dg_precision                                   */
+    double av_cav;        /* AV_CAV (0.25)                                       */
+    double av_ksensor;    /* AV_KSENSOR (0.1)                                    */
+    double eps_rho;       /* limiter floors (1e-6, :1007)                        */
+    double eps_p;
+    int    elems_per_block; /* tuning: elements per workgroup, 0 = autotuned     */
+    int    xcd_remap;       /* tuning: XCD-aware block->tile map, -1 = autotuned  */
+} dg_config;
+
+/* Initial-condition parameters (unused fields are ignored). */
+typedef struct dg_ic_params {
+    int    kind;          /* enum dg_ic_kind                                     */
+    /* circular Sod: inside (rho_hi, p_hi), outside (0.125, 0.1), radius L/4     */
+    double rho_hi, p_hi;  /* RHO_HI=11, P_HI=10                                  */
+    /* vortex: strength eps, centre (xc, yc), background velocity (u0, v0)       */
+    double vortex_eps, xc, yc, u0, v0;
+    /* Kelvin-Helmholtz: seed and amplitude of the uniform velocity noise        */
+    unsigned long long seed;
+    double noise_amp;
+} dg_ic_params;
+
+typedef struct dg_ctx dg_ctx;
+
+void        dg_config_default(dg_config* cfg);
+void        dg_ic_default(dg_ic_params* ic, int kind);
+
+/* Build LGL nodes/weights, the differentiation matrix and the workspace
+ * (replaces build_D + cudaMemcpyToSymbol, dgsem_lobatto_2d.cu:160-179,924-940,
+ * and the scratch cudaMallocs of :957-974).  `stream` is a hipStream_t (NULL =
+ * the null stream).  Every later call on the ctx is ordered on that stream. */
+int         dg_create(const dg_config* cfg, void* stream, dg_ctx** out);
+int         dg_destroy(dg_ctx* ctx);
+const char* dg_last_error(const dg_ctx* ctx);
+const char* dg_status_string(int status);
+
+/* Sizes: number of doubles in one state vector (4*NEL*NN2) and NEL. */
+long long   dg_state_size(const dg_ctx* ctx);
+long long   dg_num_elements(const dg_ctx* ctx);
+/* Copy the host-side operator tables: x[NN], w[NN], D[NN*NN] (row-major D_ik). */
+int         dg_operators(const dg_ctx* ctx, double* x, double* w, double* D);
+
+/* Evaluate the IC at the LGL nodes into device state Q, then apply the
+ * positivity limiter if it is enabled (ic_kernel + positivity_limiter,
+ * dgsem_lobatto_2d.cu:692-717,1006-1007). */
+int         dg_init_state(dg_ctx* ctx, const dg_ic_params* ic, double* Q);
+
+/* du/dt of device state Q into RHS (FP64, same layout) and the per-element
+ * max wave speed lam_el[NEL] (may be NULL).  compute_dg_rhs, :365-530. */
+int         dg_compute_rhs(dg_ctx* ctx, const double* Q, double* RHS, double* lam_el);
+
+/* One SSP-RK3 Shu-Osher stage followed by sanitize_cons and (if enabled) the
+ * positivity limiter (rk3_s1/s2/s3 + positivity_limiter, :533-576,584-643):
+ *   stage 1: Uout = U0 + dt*L
+ *   stage 2: Uout = 3/4*U0 + 1/4*(Uprev + dt*L)
+ *   stage 3: Uout = 1/3*U0 + 2/3*(Uprev + dt*L)
+ * Uprev is ignored for stage 1.  Uout may alias U0 (stage 3 of the reference
+ * writes Q while Q0 holds the copy). */
+int         dg_rk3_stage(dg_ctx* ctx, int stage, double* Uout, const double* U0,
+                         const double* Uprev, const double* L, double dt);
+
+/* Zhang-Shu positivity limiter in place (positivity_limiter, :584-643). */
+int         dg_limit(dg_ctx* ctx, double* Q);
+
+/* Max of lam_el[NEL] (device) -> *lam_max (host) and the CFL step
+ * dt = cfl*h/(lam_max*(p+1))  (reduce_max/gpu_max + :1029).  Either output
+ * pointer may be NULL.  Returns DG_ERR_NONFINITE if lam_max is not > 0. */
+int         dg_cfl_dt(dg_ctx* ctx, const double* lam_el, double cfl,
+                      double* lam_max, double* dt);
+
+/* Fused time stepping: `nsteps` SSP-RK3 steps of device state Q in place.
+ * Each stage is one kernel that computes the right-hand side and applies the
+ * Shu-Osher update (and the limiter) without writing the RHS to memory; dt is
+ * computed on the device from the wave speed of the state at the start of the
+ * step, dt = min(cfl*h/(lam_max*(p+1)), t_end - t) (the reference's loop,
+ * :1022-1058, with frame clipping replaced by one end time; pass t_end <= 0
+ * for no clipping).  No host synchronisation happens until the end.
+ * t_inout: simulation time in/out (may be NULL); dt_last: last dt (may be NULL).
+ * Returns the first numerical failure (NONFINITE / BLOWUP / DT_UNDERFLOW), in
+ * which case the state is left as it was after the last good step. */
+int         dg_advance(dg_ctx* ctx, double* Q, int nsteps, double cfl, double t_end,
+                       double* t_inout, double* dt_last);
+
+/* Same as dg_advance with nsteps = 1 (one full step of the reference loop). */
+int         dg_step(dg_ctx* ctx, double* Q, double cfl, double* dt_used);
+
+/* Per-element shock sensor theta_e (compute_sensor, :662-689), device out. */
+int         dg_sensor(dg_ctx* ctx, const double* Q, double* sensor_el);
+
+/* GLL-quadrature totals on the device, copied to host:
+ * out[0..3] = integrals of rho, rho*u, rho*v, E;  out[4] = integral of the
+ * entropy -rho*s/(gamma-1), s = ln(p*rho^-gamma). */
+int         dg_totals(dg_ctx* ctx, const double* Q, double out[5]);
+
+/* Tuning: time a few candidate schedules (elements per block, XCD remap) on
+ * scratch buffers and keep the fastest for this ctx.  Optional; dg_create
+ * picks a heuristic schedule.  Writes the chosen values if pointers non-NULL. */
+int         dg_autotune(dg_ctx* ctx, int* elems_per_block, int* xcd_remap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DGSEM_H */
