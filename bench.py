@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Benchmark of the 2-D Euler DGSEM step on one MI355X.
+
+Workload: isentropic vortex (strength 5, centre (5,5), background rho=1,
+u=v=1, p=1, gamma=1.4) on periodic [0,10]^2, 512x512 elements, degree 7
+(8x8 LGL nodes, 16.8M nodes, 67.1M DOF), Ranocha EC volume flux, Rusanov
+faces, SSP-RK3 at CFL 0.5.  A "step" is one SSP-RK3 step (three fused stage
+kernels).  The state is resident in HBM before timing starts.
+
+Prints one JSON line: value = DOF*stage updates/s of the mixed form (FP32
+volume/surface, FP64 state, RK accumulation and reductions); the FP64 form,
+the mixed/FP64 ratio, the L2/Linf error of each form against the other and
+against the exact advected vortex, the roofline of the stage kernel and the
+CPU oracle baseline are reported beside it.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--K", type=int, default=512)
+    ap.add_argument("--p", type=int, default=7)
+    ap.add_argument("--cfl", type=float, default=0.5)
+    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the error report")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_form(D, torch, cfg, ic, steps, warmup, cfl, autotune):
+    s = D.Solver(cfg)
+    tuned = s.autotune() if autotune else None
+    Q = s.init_state(ic)
+    if warmup:
+        s.advance(Q, warmup, cfl)
+    Q = s.init_state(ic, Q)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    t, dt = s.advance(Q, steps, cfl)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    return s, Q, ms, t, tuned
+
+
+def exact_vortex(cfg, t, x):
+    """Exact advected vortex at the nodes, summed over periodic images of the
+    Gaussian (the single-image form is not periodic at the 1e-6 level)."""
+    K, p = cfg.K, cfg.p
+    h = 10.0 / K
+    e = np.arange(K, dtype=np.float64)
+    X = h * (e[None, :, None, None] + 0.5 * (x[None, None, None, :] + 1.0)) + 0 * e[:, None, None, None]
+    Y = h * (e[:, None, None, None] + 0.5 * (x[None, None, :, None] + 1.0)) + 0 * e[None, :, None, None]
+    g = 1.4; eps = 5.0
+    du = np.zeros(X.shape); dv = np.zeros(X.shape); dT = np.zeros(X.shape)
+    xs = (X - t) % 10.0; ys = (Y - t) % 10.0
+    for ix in (-10.0, 0.0, 10.0):
+        for iy in (-10.0, 0.0, 10.0):
+            dx = xs - 5.0 - ix; dy = ys - 5.0 - iy
+            r2 = dx * dx + dy * dy
+            f = eps / (2 * math.pi) * np.exp(0.5 * (1 - r2))
+            du += -f * dy; dv += f * dx
+            dT += -(g - 1) * eps * eps / (8 * g * math.pi ** 2) * np.exp(1 - r2)
+    T = 1 + dT
+    r = T ** (1 / (g - 1)); pr = T ** (g / (g - 1)); u = 1 + du; v = 1 + dv
+    return np.stack([r, r * u, r * v, pr / (g - 1) + 0.5 * r * (u * u + v * v)])
+
+
+def errors(torch, A, B, wq):
+    """GLL-weighted relative L2 and relative Linf of A vs B (device tensors)."""
+    A = A.view(4, -1); B = B.view(4, -1)
+    d = A - B
+    num = torch.sqrt(torch.sum(wq * d * d)).item()
+    den = torch.sqrt(torch.sum(wq * B * B)).item()
+    return num / den, (d.abs().max() / B.abs().max()).item()
+
+
+def cpu_baseline(cfg, ic, steps, cfl):
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    o = O.Oracle(cfg)
+    Q = o.init_state(ic)
+    t0 = time.perf_counter()
+    rc, t, dt = o.advance(Q, steps, cfl)
+    wall = time.perf_counter() - t0
+    dof = o.size
+    return {"value": dof * 3 * steps / wall, "unit": "DOF*stage/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, {steps} SSP-RK3 step(s) of the same {cfg.K}x{cfg.K} p={cfg.p} vortex, "
+                      f"{threads} OpenMP threads, {wall:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
+                                   "(halo exchange of element faces), no N>1 figure is produced"}))
+        sys.exit(2)
+    import torch
+    from fvstuff_amd import dgsem as D
+    from fvstuff_amd.capi import vortex_config, make_ic
+    from oracle.oracle import lgl
+
+    ic = make_ic("vortex")
+    K, p = args.K, args.p
+    N = p + 1
+    dof = 4 * K * K * N * N
+    res = {}
+    sols = {}
+    for form in args.forms.split(","):
+        cfg = vortex_config(K=K, p=p, precision=form)
+        s, Q, ms, t, tuned = time_form(D, torch, cfg, ic, args.steps, args.warmup, args.cfl, not args.no_autotune)
+        rate = dof * 3 * args.steps / (ms * 1e-3)
+        res[form] = dict(ms=ms, rate=rate, t=t, tuned=tuned)
+        sols[form] = (s, Q, cfg)
+        log(f"[bench] {form}: {ms / args.steps:.3f} ms/step  {rate:.4e} DOF*stage/s  t={t:.6f} tuned={tuned}")
+
+    # bytes per DOF*stage: stage 1 reads U^n, writes U1 (16 B); stages 2, 3 read
+    # U_prev and U^n and write U_s (24 B)  ->  64/3 B averaged over a step
+    b_alg = 64.0 / 3.0
+    main_form = "mixed" if "mixed" in res else args.forms.split(",")[0]
+    ms_launch = res[main_form]["ms"] / (3 * args.steps)
+    achieved = b_alg * dof / (ms_launch * 1e-3) / 1e9
+
+    acc = {}
+    if not args.no_accuracy and "fp64" in sols and "mixed" in sols:
+        x, w = lgl(p)
+        h = 10.0 / K
+        wq = torch.from_numpy(np.broadcast_to(0.25 * h * h * np.outer(w, w), (K * K, N, N)).reshape(-1).copy()).cuda()
+        s64, Q64, cfg64 = sols["fp64"]
+        s32, Q32, _ = sols["mixed"]
+        t64 = res["fp64"]["t"]
+        # equal-time comparison: run the mixed form again to exactly t64
+        Q32 = s32.init_state(ic, Q32)
+        s32.advance(Q32, 10 * args.steps, args.cfl, t_end=t64)
+        l2, linf = errors(torch, Q32, Q64, wq)
+        ex = torch.from_numpy(exact_vortex(cfg64, t64, x).reshape(4, -1)).cuda()
+        e64 = errors(torch, Q64, ex, wq)
+        e32 = errors(torch, Q32, ex, wq)
+        acc = {"mixed_vs_fp64_rel_l2": l2, "mixed_vs_fp64_rel_linf": linf,
+               "fp64_vs_exact_rel_l2": e64[0], "mixed_vs_exact_rel_l2": e32[0], "t": t64}
+        T = s64.totals(Q64)
+        log(f"[bench] accuracy: {acc}")
+
+    cpu = None
+    if not args.no_cpu:
+        cfgc = vortex_config(K=K, p=p, precision="fp64")
+        cpu = cpu_baseline(cfgc, ic, args.cpu_steps, args.cfl)
+        log(f"[bench] cpu baseline: {cpu}")
+
+    out = {
+        "metric": "DOF*stage updates/s, mixed vs FP64 DGSEM at equal L2 error; % HBM peak",
+        "value": res[main_form]["rate"],
+        "unit": "DOF*stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res[main_form]["ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 RHS / fp64 state" if main_form == "mixed" else "f64",
+        "data": "synthetic (isentropic vortex IC evaluated at the LGL nodes)",
+        "config": {"workload": f"isentropic vortex, {K}x{K} elements, p={p} ({N}x{N} LGL nodes), "
+                               f"Ranocha EC volume + Rusanov faces, SSP-RK3 CFL {args.cfl}, periodic [0,10]^2",
+                   "K": K, "p": p, "dof": dof, "parallelism": "single GPU"},
+        "fp64": {"value": res.get("fp64", {}).get("rate"), "ms_per_step": res.get("fp64", {}).get("ms", 0) / args.steps},
+        "mixed_over_fp64": (res["mixed"]["rate"] / res["fp64"]["rate"]) if ("mixed" in res and "fp64" in res) else None,
+        "accuracy": acc,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"stage_kernel<{p},{'float' if main_form == 'mixed' else 'double'},lean,Ranocha>",
+                     "bytes_per_dof_stage": b_alg, "avg_launch_ms": ms_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

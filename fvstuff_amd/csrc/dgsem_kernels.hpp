@@ -56,7 +56,8 @@ struct KArgs {
     const double* w64;   // FP64 GLL weights (limiter)
     DevState* st;
     int K, NEL, E, xcd;
-    int mode, stage, vol, surf, bc, av, limiter, want_lambda, slot_cur, slot_next;
+    int mode, stage, vol, surf, bc, av, limiter, slot_cur, slot_next;
+    int check_blowup;    // stage 1: the wave speed in slot_cur came from a step of this call
     double h, gamma, ca, cb, cfl, t_end, av_cav, av_k, eps_rho, eps_p;
     int off_U, off_av, off_lam, off_el;   // byte offsets of the LDS regions
 };
@@ -221,19 +222,22 @@ __device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Rea
                 acc[i][q] += dik * F[q];
                 acc[k][q] += dki * F[q];
             }
+            // keep the scheduler from overlapping many pair fluxes: the line
+            // state and accumulators already hold (p+1)*9 values per lane
+            if (sizeof(Real) == 8 || N > 6) __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
 
-// Register budget per precision: FP32 lines keep (p+1)*9 floats live, FP64
-// twice that; the occupancy floor keeps the scheduler from trading waves for
-// instruction-level parallelism beyond what the line needs.
-template <typename Real> struct Occ;
-template <> struct Occ<float> { static constexpr int waves = 3; };
-template <> struct Occ<double> { static constexpr int waves = 2; };
+// Register budget: a line keeps its p+1 node states and 4(p+1) accumulators
+// live ((p+1)*9 values per lane, twice the registers in FP64).  The floor on
+// waves per SIMD lets FP64 at p >= 5 use the full 512-register file (VGPRs +
+// AGPRs) instead of spilling to scratch.
+template <int P, typename Real> struct Occ { static constexpr int waves = 3; };
+template <int P> struct Occ<P, double> { static constexpr int waves = P >= 5 ? 1 : 2; };
 
 template <int P, typename Real, bool FULL, int VOL>
-__global__ __launch_bounds__(BLOCK, Occ<Real>::waves) void stage_kernel(KArgs a) {
+__global__ __launch_bounds__(BLOCK, (Occ<P, Real>::waves)) void stage_kernel(KArgs a) {
     constexpr int N = P + 1, N2 = N * N;
     using Mx = M<Real>;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -258,13 +262,18 @@ __global__ __launch_bounds__(BLOCK, Occ<Real>::waves) void stage_kernel(KArgs a)
             double lam = bitsd(st->lam_bits[a.slot_cur]);
             double t = st->t;
             int status = ST_OK, skip = 0;
-            if (!(lam > 0.0)) status = ST_NONFINITE;
-            else if (lam > 1e6) status = ST_BLOWUP;
-            else {
-                if (a.t_end > 0.0 && t >= a.t_end - 1e-12 * fmax(1.0, fabs(a.t_end))) skip = 1;
+            if (a.t_end > 0.0 && t >= a.t_end - 1e-12 * fmax(1.0, fabs(a.t_end))) {
+                skip = 1;
+            } else if (!(lam > 0.0)) {
+                // NaN / non-positive speed at step start (:1028); after a step
+                // of this call it is that step's blow-up (:1054)
+                status = a.check_blowup ? ST_BLOWUP : ST_NONFINITE;
+            } else if (a.check_blowup && lam > 1e6) {
+                status = ST_BLOWUP;
+            } else {
                 dt = a.cfl * a.h / (lam * (double)N);
                 if (a.t_end > 0.0 && t + dt > a.t_end) dt = a.t_end - t;
-                if (!skip && dt < 1e-14) status = ST_DT_UNDERFLOW;
+                if (dt < 1e-14) status = ST_DT_UNDERFLOW;
             }
             if (blockIdx.x == 0 && tid == 0) {
                 st->dt = (status == ST_OK && !skip) ? dt : 0.0;
@@ -506,6 +515,7 @@ __global__ __launch_bounds__(BLOCK, Occ<Real>::waves) void stage_kernel(KArgs a)
 
     // MODE_STAGE: U_s = ca*U0 + cb*(Up + dt*R), sanitize, [limit], write, [lambda]
     const bool lim = FULL && a.limiter;
+    const bool want_lambda = a.stage == 3;
     double lmax = 0.0;
     for (int s = tid; s < nE * N2; s += nthr) {
         const int e = s / N2, nd = s - e * N2;
@@ -527,7 +537,7 @@ __global__ __launch_bounds__(BLOCK, Occ<Real>::waves) void stage_kernel(KArgs a)
         } else {
 #pragma unroll
             for (int q = 0; q < 4; q++) a.out[q * S + o] = Us[q];
-            if (a.want_lambda) {
+            if (want_lambda) {
                 double W[4];
                 Cons64::to_prim(Us, W, g);
                 lmax = nanmax(lmax, Cons64::wave_speed(W, g));
@@ -542,14 +552,14 @@ __global__ __launch_bounds__(BLOCK, Occ<Real>::waves) void stage_kernel(KArgs a)
             double Us[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) { Us[q] = sU[(size_t)e * 4 * N2 + q * N2 + nd]; a.out[q * S + o] = Us[q]; }
-            if (a.want_lambda) {
+            if (want_lambda) {
                 double W[4];
                 Cons64::to_prim(Us, W, g);
                 lmax = nanmax(lmax, Cons64::wave_speed(W, g));
             }
         }
     }
-    if (a.want_lambda) block_max_atomic(lmax, &a.st->lam_bits[a.slot_next], sred);
+    if (want_lambda) block_max_atomic(lmax, &a.st->lam_bits[a.slot_next], sred);
 }
 
 // Host-side LDS layout shared by the launcher and the kernel.

@@ -303,7 +303,7 @@ struct StageCall {
     double* out;
     double* lam_el;
     double ca, cb, cfl, t_end;
-    int want_lambda, slot_cur, slot_next;
+    int check_blowup, slot_cur, slot_next;
 };
 
 static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd) {
@@ -320,7 +320,7 @@ static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd) {
     a.st = c->d_st;
     a.K = f.K; a.NEL = c->NEL; a.E = E; a.xcd = xcd;
     a.mode = s.mode; a.stage = s.stage; a.vol = f.volume_flux; a.surf = f.surface_flux; a.bc = f.bc;
-    a.av = f.av; a.limiter = f.limiter; a.want_lambda = s.want_lambda;
+    a.av = f.av; a.limiter = f.limiter; a.check_blowup = s.check_blowup;
     a.slot_cur = s.slot_cur; a.slot_next = s.slot_next;
     a.h = c->h; a.gamma = f.gamma; a.ca = s.ca; a.cb = s.cb; a.cfl = s.cfl; a.t_end = s.t_end;
     a.av_cav = f.av_cav; a.av_k = f.av_ksensor; a.eps_rho = f.eps_rho; a.eps_p = f.eps_p;
@@ -511,12 +511,12 @@ static int advance_impl(dg_ctx* c, double* Q, int nsteps, double cfl, double t_e
         const int cur = n & 1, nxt = cur ^ 1;
         StageCall s{};
         s.mode = MODE_STAGE; s.cfl = cfl; s.t_end = t_end; s.slot_cur = cur; s.slot_next = nxt;
-        s.stage = 1; s.Up = Q; s.U0 = Q; s.out = c->d_U1; s.ca = 0.0; s.cb = 1.0; s.want_lambda = 0;
+        s.stage = 1; s.Up = Q; s.U0 = Q; s.out = c->d_U1; s.ca = 0.0; s.cb = 1.0; s.check_blowup = n > 0;
         int rc = launch_stage(c, s, E, xcd);
         if (rc) return rc;
         s.stage = 2; s.Up = c->d_U1; s.U0 = Q; s.out = c->d_U2; s.ca = 0.75; s.cb = 0.25;
         if ((rc = launch_stage(c, s, E, xcd))) return rc;
-        s.stage = 3; s.Up = c->d_U2; s.U0 = Q; s.out = Q; s.ca = 1.0 / 3.0; s.cb = 2.0 / 3.0; s.want_lambda = 1;
+        s.stage = 3; s.Up = c->d_U2; s.U0 = Q; s.out = Q; s.ca = 1.0 / 3.0; s.cb = 2.0 / 3.0;
         if ((rc = launch_stage(c, s, E, xcd))) return rc;
     }
     DevState h;
@@ -525,6 +525,12 @@ static int advance_impl(dg_ctx* c, double* Q, int nsteps, double cfl, double t_e
     if (t_inout) *t_inout = h.t;
     if (dt_last) *dt_last = h.dt;
     if (h.status != ST_OK) return fail(c, h.status, "%s at t=%.9g", dg_status_string(h.status), h.t);
+    if (nsteps > 0 && !h.skip) {   // blow-up check after the last step (:1052-1057)
+        double lam;
+        const unsigned long long b = h.lam_bits[nsteps & 1];
+        memcpy(&lam, &b, sizeof(lam));
+        if (!(lam > 0.0) || lam > 1e6) return fail(c, DG_ERR_BLOWUP, "blow-up: max wave speed %g at t=%.9g", lam, h.t);
+    }
     return DG_OK;
 }
 

@@ -1,0 +1,201 @@
+"""Host-side mirror of the DGSEM step over the C ABI (include/dgsem.h).
+
+The reference (dgsem_lobatto_2d.cu) drives its kernels from main(): build the
+operators (:924-940), set the IC (:1006-1007), then per step compute the RHS
+and wave speed (:1024-1026), pick dt (:1029), and apply the three SSP-RK3
+stages with the positivity limiter (:1040-1051).  `Solver` exposes exactly
+those operations (init_state, compute_rhs, cfl_dt, rk3_stage, limit) plus the
+fused device-side stepping (advance / step).
+
+Device memory comes from torch (float64 CUDA tensors, var-major layout
+Q[var, el, jn, in] flattened); the numerics run in libdgsem.so.  There is no
+CPU fallback: a missing library or a missing GPU raises.
+"""
+import ctypes as C
+import os
+
+import torch
+
+from .capi import DGConfig, DGICParams, STATUS_NAMES, make_config, make_ic  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdgsem.so")
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+# name -> (restype, argtypes); every function declared in include/dgsem.h
+SIGNATURES = {
+    "dg_config_default": (None, [C.POINTER(DGConfig)]),
+    "dg_ic_default": (None, [C.POINTER(DGICParams), C.c_int]),
+    "dg_create": (C.c_int, [C.POINTER(DGConfig), _vp, C.POINTER(_vp)]),
+    "dg_destroy": (C.c_int, [_vp]),
+    "dg_last_error": (C.c_char_p, [_vp]),
+    "dg_status_string": (C.c_char_p, [C.c_int]),
+    "dg_state_size": (C.c_longlong, [_vp]),
+    "dg_num_elements": (C.c_longlong, [_vp]),
+    "dg_operators": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "dg_init_state": (C.c_int, [_vp, C.POINTER(DGICParams), _vp]),
+    "dg_compute_rhs": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "dg_rk3_stage": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_double]),
+    "dg_limit": (C.c_int, [_vp, _vp]),
+    "dg_cfl_dt": (C.c_int, [_vp, _vp, C.c_double, _dp, _dp]),
+    "dg_advance": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_double, _dp, _dp]),
+    "dg_step": (C.c_int, [_vp, _vp, C.c_double, _dp]),
+    "dg_sensor": (C.c_int, [_vp, _vp, _vp]),
+    "dg_totals": (C.c_int, [_vp, _vp, _dp]),
+    "dg_autotune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+}
+
+
+class DGError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def load_library(path=LIB_PATH):
+    """Load libdgsem.so (built by fvstuff_amd.build); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built: run `python -m fvstuff_amd.build`")
+        lib = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise ValueError("expected a contiguous float64 CUDA tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+class Solver:
+    """One DGSEM discretisation (dg_ctx) on one GPU."""
+
+    def __init__(self, cfg, device="cuda:0", stream=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("fvstuff_amd.dgsem needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.lib = load_library()
+        self.cfg = cfg
+        self.device = torch.device(device)
+        torch.cuda.set_device(self.device)
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        self._ctx = C.c_void_p()
+        rc = self.lib.dg_create(C.byref(cfg), C.c_void_p(self.stream.cuda_stream), C.byref(self._ctx))
+        if rc != 0:
+            raise DGError(rc, "dg_create failed (bad config or device allocation)")
+        self.K, self.p = cfg.K, cfg.p
+        self.NN = cfg.p + 1
+        self.NEL = cfg.K * cfg.K
+        self.size = int(self.lib.dg_state_size(self._ctx))
+        self.h = cfg.L / cfg.K
+
+    def close(self):
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            self.lib.dg_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise DGError(rc, self.lib.dg_last_error(self._ctx).decode())
+
+    # ── buffers ──
+    def new_state(self):
+        return torch.empty(self.size, dtype=torch.float64, device=self.device)
+
+    def new_elem(self):
+        return torch.empty(self.NEL, dtype=torch.float64, device=self.device)
+
+    def operators(self):
+        import numpy as np
+        x = np.zeros(self.NN); w = np.zeros(self.NN); D = np.zeros(self.NN * self.NN)
+        self._check(self.lib.dg_operators(self._ctx, x.ctypes.data_as(_dp), w.ctypes.data_as(_dp),
+                                          D.ctypes.data_as(_dp)))
+        return x, w, D.reshape(self.NN, self.NN)
+
+    # ── reference operations ──
+    def init_state(self, ic, Q=None):
+        Q = self.new_state() if Q is None else Q
+        self._check(self.lib.dg_init_state(self._ctx, C.byref(ic), _ptr(Q)))
+        return Q
+
+    def compute_rhs(self, Q, RHS=None, lam=None, want_lambda=True):
+        RHS = self.new_state() if RHS is None else RHS
+        if lam is None and want_lambda:
+            lam = self.new_elem()
+        self._check(self.lib.dg_compute_rhs(self._ctx, _ptr(Q), _ptr(RHS), _ptr(lam)))
+        return RHS, lam
+
+    def rk3_stage(self, stage, U0, Uprev, L, dt, out=None):
+        out = self.new_state() if out is None else out
+        self._check(self.lib.dg_rk3_stage(self._ctx, int(stage), _ptr(out), _ptr(U0), _ptr(Uprev), _ptr(L),
+                                          float(dt)))
+        return out
+
+    def limit(self, Q):
+        self._check(self.lib.dg_limit(self._ctx, _ptr(Q)))
+        return Q
+
+    def cfl_dt(self, lam, cfl):
+        lm = C.c_double(); dt = C.c_double()
+        self._check(self.lib.dg_cfl_dt(self._ctx, _ptr(lam), float(cfl), C.byref(lm), C.byref(dt)))
+        return lm.value, dt.value
+
+    # ── fused stepping ──
+    def advance(self, Q, nsteps, cfl, t_end=0.0, t=0.0):
+        """nsteps fused SSP-RK3 steps in place; returns (t, dt_last)."""
+        tt = C.c_double(t); dt = C.c_double(0.0)
+        self._check(self.lib.dg_advance(self._ctx, _ptr(Q), int(nsteps), float(cfl), float(t_end),
+                                        C.byref(tt), C.byref(dt)))
+        return tt.value, dt.value
+
+    def step(self, Q, cfl):
+        dt = C.c_double(0.0)
+        self._check(self.lib.dg_step(self._ctx, _ptr(Q), float(cfl), C.byref(dt)))
+        return dt.value
+
+    def reference_step(self, Q, cfl, scratch=None):
+        """One step as the reference's loop issues it (:1024-1051): separate
+        RHS, dt, stage and limiter calls.  Returns dt."""
+        Q0, Q1, Q2, R, lam = scratch if scratch is not None else (
+            self.new_state(), self.new_state(), self.new_state(), self.new_state(), self.new_elem())
+        self.compute_rhs(Q, R, lam)
+        _, dt = self.cfl_dt(lam, cfl)
+        Q0.copy_(Q)
+        self.rk3_stage(1, Q0, None, R, dt, out=Q1)
+        self.compute_rhs(Q1, R, lam)
+        self.rk3_stage(2, Q0, Q1, R, dt, out=Q2)
+        self.compute_rhs(Q2, R, lam)
+        self.rk3_stage(3, Q0, Q2, R, dt, out=Q)
+        return dt
+
+    # ── diagnostics ──
+    def sensor(self, Q):
+        s = self.new_elem()
+        self._check(self.lib.dg_sensor(self._ctx, _ptr(Q), _ptr(s)))
+        return s
+
+    def totals(self, Q):
+        out = (C.c_double * 5)()
+        self._check(self.lib.dg_totals(self._ctx, _ptr(Q), out))
+        return list(out)
+
+    def autotune(self):
+        E = C.c_int(); X = C.c_int()
+        self._check(self.lib.dg_autotune(self._ctx, C.byref(E), C.byref(X)))
+        return E.value, X.value

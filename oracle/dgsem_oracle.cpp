@@ -740,7 +740,7 @@ OR_API int oracle_advance(void* ctx, double* Q, int nsteps, double cfl, double t
     size_t n = (size_t)4 * c.NEL * c.NN2;
     double t = t_inout ? *t_inout : 0.0;
     for (int s = 0; s < nsteps; s++) {
-        if (t_end > 0.0 && t >= t_end) break;
+        if (t_end > 0.0 && t >= t_end - 1e-12 * std::max(1.0, fabs(t_end))) break;
         compute_rhs(c, Q, c.RHS.data(), c.lam.data());
         double lam_max = max_lambda(c, c.lam.data());
         if (!(lam_max > 0.0)) return DG_ERR_NONFINITE;

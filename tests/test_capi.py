@@ -1,0 +1,68 @@
+"""The C-ABI library loads and exports every function include/dgsem.h
+declares, and the ctypes mirror matches the C struct layout.  No device calls."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from fvstuff_amd import capi
+from fvstuff_amd.dgsem import LIB_PATH, SIGNATURES, load_library
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dgsem.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(dg_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_and_wrapper_agree():
+    assert declared_functions() == sorted(SIGNATURES)
+
+
+def test_library_exports_every_symbol():
+    assert os.path.exists(LIB_PATH), "libdgsem.so not built (python -m fvstuff_amd.build)"
+    lib = load_library()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True).stdout
+    for name in declared_functions():
+        assert re.search(rf"\bT {name}$", out, re.M), name
+
+
+def test_host_only_calls():
+    lib = load_library()
+    cfg = capi.DGConfig()
+    lib.dg_config_default(C.byref(cfg))
+    ref = capi.make_config()
+    for f, _ in capi.DGConfig._fields_:
+        assert getattr(cfg, f) == getattr(ref, f), f
+    assert lib.dg_status_string(5).decode().startswith("blow-up")
+    ic = capi.DGICParams()
+    lib.dg_ic_default(C.byref(ic), 1)
+    assert ic.kind == 1 and ic.vortex_eps == 5.0 and ic.xc == 5.0 and ic.u0 == 1.0
+
+
+def test_struct_layout_matches_header(tmp_path):
+    prog = tmp_path / "layout.c"
+    fields_cfg = [f for f, _ in capi.DGConfig._fields_]
+    fields_ic = [f for f, _ in capi.DGICParams._fields_]
+    body = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void){']
+    body.append('printf("%zu %zu\\n", sizeof(dg_config), sizeof(dg_ic_params));')
+    for f in fields_cfg:
+        body.append(f'printf("%zu\\n", offsetof(dg_config, {f}));')
+    for f in fields_ic:
+        body.append(f'printf("%zu\\n", offsetof(dg_ic_params, {f}));')
+    body.append("return 0;}")
+    prog.write_text("\n".join(body))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-o", str(exe), str(prog)])
+    vals = subprocess.check_output([str(exe)]).decode().split()
+    assert int(vals[0]) == C.sizeof(capi.DGConfig) and int(vals[1]) == C.sizeof(capi.DGICParams)
+    offs = [int(v) for v in vals[2:]]
+    exp = [getattr(capi.DGConfig, f).offset for f in fields_cfg] + [getattr(capi.DGICParams, f).offset for f in fields_ic]
+    assert offs == exp
