@@ -35,6 +35,9 @@ def build(jobs=None, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     units = [(os.path.join(CSRC, "dgsem_inst.hip"), os.path.join(BUILD, f"dgsem_p{p}.o"), [f"-DDG_P={p}"])
              for p in DEGREES]
+    for real in ("float", "double"):
+        units.append((os.path.join(CSRC, "dgsem_batch.hip"), os.path.join(BUILD, f"dgsem_batch_{real}.o"),
+                      [f"-DDG_REAL={real}"]))
     units.append((os.path.join(CSRC, "dgsem_lib.hip"), os.path.join(BUILD, "dgsem_lib.o"), []))
     cmds = [[HIPCC, *FLAGS, *extra, "-c", src, "-o", obj] for src, obj, extra in units]
     jobs = jobs or min(8, os.cpu_count() or 4)

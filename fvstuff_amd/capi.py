@@ -98,3 +98,24 @@ def kh_config(K=64, p=3, precision="mixed", **kw):
                 precision=precision)
     args.update(kw)
     return make_config(**args)
+
+
+def ensemble(n=256, seed=2024, precision="mixed", k_range=(16, 128), degrees=(3, 5, 7),
+             eps_range=(1.0, 5.0), mach_range=(0.1, 0.8), gamma=1.4):
+    """Seed-drawn isentropic-vortex instances: K ~ U{k_range}, p ~ U{degrees},
+    strength ~ U[eps_range], background Mach ~ U[mach_range] along the diagonal
+    (|(u0,v0)| = M*sqrt(gamma) for the unit background sound speed).
+    Returns (configs, ics, cfls); the CFL follows the degree's SSP-RK3 limit."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    cfgs, ics, cfls = [], [], []
+    for _ in range(n):
+        K = int(rng.integers(k_range[0], k_range[1] + 1))
+        p = int(rng.choice(degrees))
+        eps = float(rng.uniform(*eps_range))
+        M = float(rng.uniform(*mach_range))
+        speed = M * gamma ** 0.5 / 2 ** 0.5
+        cfgs.append(vortex_config(K=K, p=p, precision=precision))
+        ics.append(make_ic("vortex", vortex_eps=eps, u0=speed, v0=speed))
+        cfls.append(0.5 if p <= 3 else (0.35 if p <= 5 else 0.25))
+    return cfgs, ics, cfls

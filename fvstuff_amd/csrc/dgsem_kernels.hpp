@@ -55,11 +55,16 @@ struct KArgs {
     const void* tab;     // compute-precision tables: D[N*N], w[N], winv[N]
     const double* w64;   // FP64 GLL weights (limiter)
     DevState* st;
-    int K, NEL, E, xcd;
+    int P, K, NEL, E, xcd;
     int mode, stage, vol, surf, bc, av, limiter, slot_cur, slot_next;
     int check_blowup;    // stage 1: the wave speed in slot_cur came from a step of this call
     double h, gamma, ca, cb, cfl, t_end, av_cav, av_k, eps_rho, eps_p;
     int off_U, off_av, off_lam, off_el;   // byte offsets of the LDS regions
+};
+
+// Per-launch step control of the fused stepping (ping-pong wave-speed slots).
+struct StepCtl {
+    int slot_cur, slot_next, check_blowup;
 };
 
 __device__ __forceinline__ unsigned long long dbits(double x) {
@@ -236,14 +241,17 @@ __device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Rea
 template <int P, typename Real> struct Occ { static constexpr int waves = 3; };
 template <int P> struct Occ<P, double> { static constexpr int waves = P >= 5 ? 1 : 2; };
 
+// One tile (E consecutive elements) of one stage.  `lead` marks the one tile
+// per launch that updates the device step state.  Every early exit is taken
+// by the whole block (decided by thread 0, broadcast through LDS).
 template <int P, typename Real, bool FULL, int VOL>
-__global__ __launch_bounds__(BLOCK, (Occ<P, Real>::waves)) void stage_kernel(KArgs a) {
+__device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int tile, bool lead) {
     constexpr int N = P + 1, N2 = N * N;
     using Mx = M<Real>;
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double s_dt;
+    __shared__ int s_go;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int G = gridDim.x;
-    const int tile = tile_of_block(blockIdx.x, G, a.xcd);
     const int E = a.E;
     const int el0 = tile * E;
     const int nE = min(E, a.NEL - el0);
@@ -256,39 +264,50 @@ __global__ __launch_bounds__(BLOCK, (Occ<P, Real>::waves)) void stage_kernel(KAr
     // ── fused dt (device side, :1026-1035) ──
     double dt = 0.0;
     if (a.mode == MODE_STAGE) {
-        DevState* st = a.st;
-        if (st->status != ST_OK) return;
-        if (a.stage == 1) {
-            double lam = bitsd(st->lam_bits[a.slot_cur]);
-            double t = st->t;
-            int status = ST_OK, skip = 0;
-            if (a.t_end > 0.0 && t >= a.t_end - 1e-12 * fmax(1.0, fabs(a.t_end))) {
-                skip = 1;
-            } else if (!(lam > 0.0)) {
-                // NaN / non-positive speed at step start (:1028); after a step
-                // of this call it is that step's blow-up (:1054)
-                status = a.check_blowup ? ST_BLOWUP : ST_NONFINITE;
-            } else if (a.check_blowup && lam > 1e6) {
-                status = ST_BLOWUP;
-            } else {
-                dt = a.cfl * a.h / (lam * (double)N);
-                if (a.t_end > 0.0 && t + dt > a.t_end) dt = a.t_end - t;
-                if (dt < 1e-14) status = ST_DT_UNDERFLOW;
+        __syncthreads();   // a persistent loop's previous tile is done with s_go / s_dt / LDS
+        if (tid == 0) {
+            DevState* st = a.st;
+            int go = 0;
+            double d = 0.0;
+            if (st->status == ST_OK) {
+                if (a.stage == 1) {
+                    const double lam = bitsd(st->lam_bits[sc.slot_cur]);
+                    const double t = st->t;
+                    int status = ST_OK, skip = 0;
+                    if (a.t_end > 0.0 && t >= a.t_end - 1e-12 * fmax(1.0, fabs(a.t_end))) {
+                        skip = 1;
+                    } else if (!(lam > 0.0)) {
+                        // NaN / non-positive speed at step start (:1028); after a
+                        // step of this call it is that step's blow-up (:1054)
+                        status = sc.check_blowup ? ST_BLOWUP : ST_NONFINITE;
+                    } else if (sc.check_blowup && lam > 1e6) {
+                        status = ST_BLOWUP;
+                    } else {
+                        d = a.cfl * a.h / (lam * (double)N);
+                        if (a.t_end > 0.0 && t + d > a.t_end) d = a.t_end - t;
+                        if (d < 1e-14) status = ST_DT_UNDERFLOW;
+                    }
+                    go = status == ST_OK && !skip;
+                    if (lead) {
+                        st->dt = go ? d : 0.0;
+                        st->skip = skip;
+                        if (status != ST_OK) st->status = status;
+                    }
+                } else if (!st->skip) {
+                    d = st->dt;
+                    go = 1;
+                    if (a.stage == 2 && lead) {
+                        st->t += d;
+                        st->lam_bits[sc.slot_next] = 0ull;
+                    }
+                }
             }
-            if (blockIdx.x == 0 && tid == 0) {
-                st->dt = (status == ST_OK && !skip) ? dt : 0.0;
-                st->skip = skip;
-                if (status != ST_OK) st->status = status;
-            }
-            if (status != ST_OK || skip) return;
-        } else {
-            if (st->skip) return;
-            dt = st->dt;
-            if (a.stage == 2 && blockIdx.x == 0 && tid == 0) {
-                st->t += dt;
-                st->lam_bits[a.slot_next] = 0ull;
-            }
+            s_go = go;
+            s_dt = d;
         }
+        __syncthreads();
+        if (!s_go) return;
+        dt = s_dt;
     }
 
     const Phys<Real> ph{(Real)g, (Real)(g - 1.0), (Real)(1.0 / (g - 1.0))};
@@ -559,7 +578,40 @@ __global__ __launch_bounds__(BLOCK, (Occ<P, Real>::waves)) void stage_kernel(KAr
             }
         }
     }
-    if (want_lambda) block_max_atomic(lmax, &a.st->lam_bits[a.slot_next], sred);
+    if (want_lambda) block_max_atomic(lmax, &a.st->lam_bits[sc.slot_next], sred);
+}
+
+template <int P, typename Real, bool FULL, int VOL>
+__global__ __launch_bounds__(BLOCK, (Occ<P, Real>::waves)) void stage_kernel(KArgs a) {
+    const StepCtl sc{a.slot_cur, a.slot_next, a.check_blowup};
+    stage_tile<P, Real, FULL, VOL>(a, sc, tile_of_block(blockIdx.x, gridDim.x, a.xcd), blockIdx.x == 0);
+}
+
+// Persistent batched stage: many independent solves (own K, p, domain, dt)
+// in one launch.  tiles[] lists (instance, tile) pairs, heaviest first; each
+// workgroup walks the list with a grid stride.  args[3*i + stage-1] holds the
+// instance's stage arguments (its own device step state and wave-speed slots).
+// The batched kernel carries every degree's body; its register budget is the
+// largest one's (2 waves/SIMD in FP32, 1 in FP64).
+template <typename Real> struct BatchOcc { static constexpr int waves = sizeof(Real) == 4 ? 2 : 1; };
+
+template <typename Real, bool FULL, int VOL>
+__global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_kernel(
+    const KArgs* __restrict__ args, const int2* __restrict__ tiles, int ntiles, int stage, StepCtl sc) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 tl = tiles[t];
+        const KArgs& a = args[(size_t)tl.x * 3 + (stage - 1)];
+        const bool lead = tl.y == 0;
+        switch (a.P) {
+            case 1: stage_tile<1, Real, FULL, VOL>(a, sc, tl.y, lead); break;
+            case 2: stage_tile<2, Real, FULL, VOL>(a, sc, tl.y, lead); break;
+            case 3: stage_tile<3, Real, FULL, VOL>(a, sc, tl.y, lead); break;
+            case 4: stage_tile<4, Real, FULL, VOL>(a, sc, tl.y, lead); break;
+            case 5: stage_tile<5, Real, FULL, VOL>(a, sc, tl.y, lead); break;
+            case 6: stage_tile<6, Real, FULL, VOL>(a, sc, tl.y, lead); break;
+            default: stage_tile<7, Real, FULL, VOL>(a, sc, tl.y, lead); break;
+        }
+    }
 }
 
 // Host-side LDS layout shared by the launcher and the kernel.

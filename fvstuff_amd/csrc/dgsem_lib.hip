@@ -318,7 +318,7 @@ static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd) {
     a.tab = fp32 ? c->d_tab32 : c->d_tab64;
     a.w64 = (const double*)c->d_tab64 + c->N2;
     a.st = c->d_st;
-    a.K = f.K; a.NEL = c->NEL; a.E = E; a.xcd = xcd;
+    a.P = f.p; a.K = f.K; a.NEL = c->NEL; a.E = E; a.xcd = xcd;
     a.mode = s.mode; a.stage = s.stage; a.vol = f.volume_flux; a.surf = f.surface_flux; a.bc = f.bc;
     a.av = f.av; a.limiter = f.limiter; a.check_blowup = s.check_blowup;
     a.slot_cur = s.slot_cur; a.slot_next = s.slot_next;
@@ -329,6 +329,46 @@ static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd) {
     const int block = std::min(BLOCK, (E * 2 * c->N + 63) / 64 * 64);
     HIPCK(c, stage_launcher(f.p)(a, fp32, full, grid, block, (size_t)L.total, c->stream));
     return DG_OK;
+}
+
+// ── batched-instance helpers ──────────────────────────────────────────────
+// args[3*i] is instance i's stage-1 argument block (its state pointer,
+// sizes and device step state).
+__global__ void batch_reset_kernel(const KArgs* args, int n, const double* t0) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    DevState* st = args[3 * i].st;
+    st->lam_bits[0] = 0ull;
+    st->lam_bits[1] = 0ull;
+    st->t = t0 ? t0[i] : 0.0;
+    st->dt = 0.0;
+    st->status = ST_OK;
+    st->skip = 0;
+}
+
+// wave speed of each instance's state, one tile of E elements per visit
+__global__ void batch_lambda_kernel(const KArgs* args, const int2* tiles, int ntiles) {
+    __shared__ double sred[BLOCK / 64];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 tl = tiles[t];
+        const KArgs& a = args[(size_t)tl.x * 3];
+        const int N2 = (a.P + 1) * (a.P + 1);
+        const size_t S = (size_t)a.NEL * N2;
+        const int el0 = tl.y * a.E, nE = min(a.E, a.NEL - el0);
+        double m = 0.0;
+        for (int s = threadIdx.x; s < nE * N2; s += blockDim.x) {
+            double W[4];
+            load_prim64(a.Up, S, (size_t)el0 * N2 + s, a.gamma, W);
+            m = nanmax(m, Cons64::wave_speed(W, a.gamma));
+        }
+        __syncthreads();   // sred reuse across tiles
+        block_max_atomic(m, &a.st->lam_bits[0], sred);
+    }
+}
+
+__global__ void batch_gather_kernel(const KArgs* args, int n, DevState* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = *args[3 * i].st;
 }
 
 // ── exported API ───────────────────────────────────────────────────────────
@@ -610,6 +650,299 @@ int dg_autotune(dg_ctx* c, int* elems_per_block, int* xcd_remap) {
     if (elems_per_block) *elems_per_block = bestE;
     if (xcd_remap) *xcd_remap = bestX;
     return DG_OK;
+}
+
+// ── batched instances ─────────────────────────────────────────────────────
+}  // extern "C"
+
+struct dg_batch {
+    int n = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t cap = nullptr;          // capture / replay stream (default priority)
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    std::vector<dg_ctx*> ctx;
+    dg_config common;
+    std::vector<KArgs> h_args;          // 3 per instance
+    KArgs* d_args = nullptr;
+    int2* d_tiles = nullptr;
+    DevState* d_gather = nullptr;
+    double* d_t0 = nullptr;
+    int ntiles = 0, grid = 0, grid_override = 0, occ_blocks = 1, cus = 1;
+    size_t lds = 0;
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    char err[512] = {0};
+};
+
+static int bfail(dg_batch* b, int code, const char* fmt, ...) {
+    if (b) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(b->err, sizeof(b->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+#define BHIPCK(b, x)                                                                          \
+    do {                                                                                      \
+        hipError_t _e = (x);                                                                  \
+        if (_e != hipSuccess)                                                                 \
+            return bfail((b), DG_ERR_HIP, "%s:%d: %s", __FILE__, __LINE__, hipGetErrorString(_e)); \
+    } while (0)
+
+static hipError_t batch_launch_stage(dg_batch* b, int stage, StepCtl sc) {
+    const bool fp32 = b->common.precision == DG_MIXED;
+    return fp32 ? launch_batch_stage_float(b->d_args, b->d_tiles, b->ntiles, stage, sc, b->common.volume_flux,
+                                           b->grid, b->lds, b->cap)
+                : launch_batch_stage_double(b->d_args, b->d_tiles, b->ntiles, stage, sc, b->common.volume_flux,
+                                            b->grid, b->lds, b->cap);
+}
+
+static hipError_t batch_step(dg_batch* b, StepCtl sc) {
+    hipError_t e;
+    for (int stage = 1; stage <= 3; stage++)
+        if ((e = batch_launch_stage(b, stage, sc)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
+// Fill the three per-stage argument blocks of every instance for states Qs.
+static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double t_end) {
+    for (int i = 0; i < b->n; i++) {
+        dg_ctx* c = b->ctx[i];
+        const dg_config& f = c->cfg;
+        const bool fp32 = f.precision == DG_MIXED;
+        LdsLayout L = lds_layout(c->N, max_elems_per_block(c->N), fp32 ? 4 : 8, false, false, false);
+        for (int s = 1; s <= 3; s++) {
+            KArgs a;
+            memset(&a, 0, sizeof(a));
+            a.Up = s == 1 ? Qs[i] : (s == 2 ? c->d_U1 : c->d_U2);
+            a.U0 = Qs[i];
+            a.out = s == 1 ? c->d_U1 : (s == 2 ? c->d_U2 : Qs[i]);
+            a.lam_el = nullptr;
+            a.tab = fp32 ? c->d_tab32 : c->d_tab64;
+            a.w64 = (const double*)c->d_tab64 + c->N2;
+            a.st = c->d_st;
+            a.P = f.p; a.K = f.K; a.NEL = c->NEL; a.E = max_elems_per_block(c->N); a.xcd = 0;
+            a.mode = MODE_STAGE; a.stage = s; a.vol = f.volume_flux; a.surf = f.surface_flux; a.bc = f.bc;
+            a.av = 0; a.limiter = 0;
+            a.h = c->h; a.gamma = f.gamma;
+            a.ca = s == 1 ? 0.0 : (s == 2 ? 0.75 : 1.0 / 3.0);
+            a.cb = s == 1 ? 1.0 : (s == 2 ? 0.25 : 2.0 / 3.0);
+            a.cfl = cfl[i]; a.t_end = t_end;
+            a.av_cav = f.av_cav; a.av_k = f.av_ksensor; a.eps_rho = f.eps_rho; a.eps_p = f.eps_p;
+            a.off_U = L.off_U; a.off_av = L.off_av; a.off_lam = L.off_lam; a.off_el = L.off_el;
+            b->h_args[(size_t)i * 3 + (s - 1)] = a;
+        }
+    }
+}
+
+static int batch_status(dg_batch* b, int nsteps, double* t_inout, int* status_out) {
+    std::vector<DevState> h(b->n);
+    hipLaunchKernelGGL(batch_gather_kernel, dim3((b->n + 63) / 64), dim3(64), 0, b->stream, b->d_args, b->n,
+                       b->d_gather);
+    BHIPCK(b, hipGetLastError());
+    BHIPCK(b, hipMemcpyAsync(h.data(), b->d_gather, b->n * sizeof(DevState), hipMemcpyDeviceToHost, b->stream));
+    BHIPCK(b, hipStreamSynchronize(b->stream));
+    int first = DG_OK;
+    for (int i = 0; i < b->n; i++) {
+        int st = h[i].status;
+        if (st == ST_OK && nsteps > 0 && !h[i].skip) {
+            double lam;
+            memcpy(&lam, &h[i].lam_bits[nsteps & 1], sizeof(lam));
+            if (!(lam > 0.0) || lam > 1e6) st = DG_ERR_BLOWUP;
+        }
+        if (t_inout) t_inout[i] = h[i].t;
+        if (status_out) status_out[i] = st;
+        if (st != DG_OK && first == DG_OK) {
+            first = st;
+            bfail(b, st, "instance %d: %s at t=%.9g", i, dg_status_string(st), h[i].t);
+        }
+    }
+    return first;
+}
+
+extern "C" {
+
+int dg_batch_create(const dg_config* cfgs, int n, void* stream, dg_batch** out) {
+    if (!cfgs || n < 1 || !out) return DG_ERR_INVALID_ARG;
+    *out = nullptr;
+    const dg_config& c0 = cfgs[0];
+    for (int i = 0; i < n; i++) {
+        const dg_config& f = cfgs[i];
+        if (f.volume_flux != c0.volume_flux || f.surface_flux != c0.surface_flux || f.bc != c0.bc ||
+            f.precision != c0.precision || f.av || f.limiter)
+            return DG_ERR_INVALID_ARG;
+    }
+    dg_batch* b = new dg_batch();
+    b->n = n;
+    b->stream = (hipStream_t)stream;
+    b->common = c0;
+    b->ctx.assign(n, nullptr);
+    for (int i = 0; i < n; i++) {
+        int rc = dg_create(&cfgs[i], stream, &b->ctx[i]);
+        if (rc) { dg_batch_destroy(b); return rc; }
+    }
+    // tile list, heaviest first: cost of an element ~ (p+1)^3 (pairs per line x lines)
+    std::vector<std::pair<double, int2>> tl;
+    size_t lds = 0;
+    for (int i = 0; i < n; i++) {
+        dg_ctx* c = b->ctx[i];
+        const int E = max_elems_per_block(c->N);
+        const int nt = (c->NEL + E - 1) / E;
+        const double cost = (double)E * c->N * c->N * c->N;
+        for (int t = 0; t < nt; t++) tl.push_back({cost, make_int2(i, t)});
+        LdsLayout L = lds_layout(c->N, E, c0.precision == DG_MIXED ? 4 : 8, false, false, false);
+        lds = std::max(lds, (size_t)L.total);
+    }
+    std::stable_sort(tl.begin(), tl.end(), [](const std::pair<double, int2>& x, const std::pair<double, int2>& y) {
+        return x.first > y.first;
+    });
+    std::vector<int2> tiles(tl.size());
+    for (size_t k = 0; k < tl.size(); k++) tiles[k] = tl[k].second;
+    b->ntiles = (int)tiles.size();
+    b->lds = lds;
+    b->h_args.resize((size_t)3 * n);
+#define BCK(x)                                                                 \
+    do {                                                                       \
+        if ((x) != hipSuccess) { dg_batch_destroy(b); return DG_ERR_HIP; }    \
+    } while (0)
+    BCK(hipMalloc(&b->d_args, (size_t)3 * n * sizeof(KArgs)));
+    BCK(hipMalloc(&b->d_tiles, tiles.size() * sizeof(int2)));
+    BCK(hipMalloc(&b->d_gather, (size_t)n * sizeof(DevState)));
+    BCK(hipMalloc(&b->d_t0, (size_t)n * sizeof(double)));
+    BCK(hipMemcpyAsync(b->d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream));
+    BCK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    BCK(hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming));
+    BCK(hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming));
+    int dev = 0;
+    BCK(hipGetDevice(&dev));
+    BCK(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev));
+    BCK(c0.precision == DG_MIXED ? batch_stage_occupancy_float(c0.volume_flux, lds, &b->occ_blocks)
+                                 : batch_stage_occupancy_double(c0.volume_flux, lds, &b->occ_blocks));
+    BCK(hipStreamSynchronize(b->stream));
+#undef BCK
+    b->occ_blocks = std::max(1, b->occ_blocks);
+    b->grid = std::min(b->ntiles, b->cus * b->occ_blocks);
+    *out = b;
+    return DG_OK;
+}
+
+int dg_batch_destroy(dg_batch* b) {
+    if (!b) return DG_OK;
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    for (dg_ctx* c : b->ctx) dg_destroy(c);
+    hipFree(b->d_args); hipFree(b->d_tiles); hipFree(b->d_gather); hipFree(b->d_t0);
+    if (b->cap) hipStreamDestroy(b->cap);
+    if (b->ev_in) hipEventDestroy(b->ev_in);
+    if (b->ev_out) hipEventDestroy(b->ev_out);
+    delete b;
+    return DG_OK;
+}
+
+const char* dg_batch_last_error(const dg_batch* b) { return b ? b->err : "null batch"; }
+
+long long dg_batch_state_size(const dg_batch* b, int i) {
+    return (b && i >= 0 && i < b->n) ? dg_state_size(b->ctx[i]) : 0;
+}
+
+int dg_batch_init_state(dg_batch* b, const dg_ic_params* ics, double* const* Qs) {
+    if (!b || !ics || !Qs) return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_init_state: null argument");
+    for (int i = 0; i < b->n; i++) {
+        int rc = dg_init_state(b->ctx[i], &ics[i], Qs[i]);
+        if (rc) return bfail(b, rc, "instance %d: %s", i, b->ctx[i]->err);
+    }
+    return DG_OK;
+}
+
+int dg_batch_set_options(dg_batch* b, int use_graph, int grid_blocks) {
+    if (!b || grid_blocks < 0) return DG_ERR_INVALID_ARG;
+    b->use_graph = use_graph != 0;
+    b->grid_override = grid_blocks;
+    b->grid = grid_blocks > 0 ? std::min(grid_blocks, b->ntiles) : std::min(b->ntiles, b->cus * b->occ_blocks);
+    if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }   // grid is baked into the graph
+    return DG_OK;
+}
+
+int dg_batch_advance(dg_batch* b, double* const* Qs, int nsteps, const double* cfl, double t_end, double* t_inout,
+                     int* status_out) {
+    if (!b || !Qs || !cfl || nsteps < 0) return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_advance: bad argument");
+    for (int i = 0; i < b->n; i++)
+        if (!Qs[i] || !(cfl[i] > 0)) return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_advance: instance %d", i);
+    batch_bind(b, Qs, cfl, t_end);
+    BHIPCK(b, hipMemcpyAsync(b->d_args, b->h_args.data(), b->h_args.size() * sizeof(KArgs), hipMemcpyHostToDevice,
+                             b->stream));
+    if (t_inout) BHIPCK(b, hipMemcpyAsync(b->d_t0, t_inout, b->n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(batch_reset_kernel, dim3((b->n + 63) / 64), dim3(64), 0, b->stream, b->d_args, b->n,
+                       t_inout ? b->d_t0 : nullptr);
+    hipLaunchKernelGGL(batch_lambda_kernel, dim3(b->grid), dim3(BLOCK), 0, b->stream, b->d_args, b->d_tiles,
+                       b->ntiles);
+    BHIPCK(b, hipGetLastError());
+    // stage launches run on the capture stream, ordered after the work above
+    BHIPCK(b, hipEventRecord(b->ev_in, b->stream));
+    BHIPCK(b, hipStreamWaitEvent(b->cap, b->ev_in, 0));
+    if (nsteps > 0) BHIPCK(b, batch_step(b, StepCtl{0, 1, 0}));
+    int n = 1;
+    if (b->use_graph && nsteps >= 3) {
+        if (!b->gexec) {   // two steps (odd, even) per replay; arguments live in d_args
+            hipGraph_t g;
+            BHIPCK(b, hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+            hipError_t e1 = batch_step(b, StepCtl{1, 0, 1});
+            hipError_t e2 = batch_step(b, StepCtl{0, 1, 1});
+            hipError_t ec = hipStreamEndCapture(b->cap, &g);
+            BHIPCK(b, e1); BHIPCK(b, e2); BHIPCK(b, ec);
+            hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            BHIPCK(b, ei);
+        }
+        for (; n + 1 < nsteps; n += 2) BHIPCK(b, hipGraphLaunch(b->gexec, b->cap));
+    }
+    for (; n < nsteps; n++) BHIPCK(b, batch_step(b, StepCtl{n & 1, (n & 1) ^ 1, 1}));
+    BHIPCK(b, hipEventRecord(b->ev_out, b->cap));
+    BHIPCK(b, hipStreamWaitEvent(b->stream, b->ev_out, 0));
+    return batch_status(b, nsteps, t_inout, status_out);
+}
+
+int dg_batch_advance_each(dg_batch* b, double* const* Qs, int nsteps, const double* cfl, double t_end,
+                          double* t_inout, int* status_out) {
+    if (!b || !Qs || !cfl || nsteps < 0) return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_advance_each: bad argument");
+    int first = DG_OK;
+    for (int i = 0; i < b->n; i++) {
+        double t = t_inout ? t_inout[i] : 0.0;
+        int rc = dg_advance(b->ctx[i], Qs[i], nsteps, cfl[i], t_end, &t, nullptr);
+        if (t_inout) t_inout[i] = t;
+        if (status_out) status_out[i] = rc;
+        if (rc && first == DG_OK) { first = rc; bfail(b, rc, "instance %d: %s", i, b->ctx[i]->err); }
+    }
+    return first;
+}
+
+int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* grid_blocks) {
+    if (!b || !Qs || !cfl) return DG_ERR_INVALID_ARG;
+    const int base = b->cus * b->occ_blocks;
+    const int cands[] = {base, base / 2, 2 * base, 4 * base};
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    float best = 1e30f;
+    int best_grid = b->grid;
+    int rc = DG_OK;
+    for (int g : cands) {
+        if (g < 1) continue;
+        dg_batch_set_options(b, b->use_graph, g);
+        if ((rc = dg_batch_advance(b, Qs, 1, cfl, 0.0, nullptr, nullptr))) break;   // warm-up + graph
+        hipEventRecord(e0, b->stream);
+        if ((rc = dg_batch_advance(b, Qs, 3, cfl, 0.0, nullptr, nullptr))) break;
+        hipEventRecord(e1, b->stream);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) { best = ms; best_grid = g; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    dg_batch_set_options(b, b->use_graph, best_grid);
+    if (grid_blocks) *grid_blocks = b->grid;
+    return rc;
 }
 
 }  // extern "C"

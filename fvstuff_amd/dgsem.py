@@ -46,6 +46,15 @@ SIGNATURES = {
     "dg_sensor": (C.c_int, [_vp, _vp, _vp]),
     "dg_totals": (C.c_int, [_vp, _vp, _dp]),
     "dg_autotune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "dg_batch_create": (C.c_int, [C.POINTER(DGConfig), C.c_int, _vp, C.POINTER(_vp)]),
+    "dg_batch_destroy": (C.c_int, [_vp]),
+    "dg_batch_last_error": (C.c_char_p, [_vp]),
+    "dg_batch_state_size": (C.c_longlong, [_vp, C.c_int]),
+    "dg_batch_init_state": (C.c_int, [_vp, C.POINTER(DGICParams), C.POINTER(_vp)]),
+    "dg_batch_advance": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, _dp, C.c_double, _dp, C.POINTER(C.c_int)]),
+    "dg_batch_advance_each": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, _dp, C.c_double, _dp, C.POINTER(C.c_int)]),
+    "dg_batch_set_options": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "dg_batch_autotune": (C.c_int, [_vp, C.POINTER(_vp), _dp, C.POINTER(C.c_int)]),
 }
 
 
@@ -199,3 +208,81 @@ class Solver:
         E = C.c_int(); X = C.c_int()
         self._check(self.lib.dg_autotune(self._ctx, C.byref(E), C.byref(X)))
         return E.value, X.value
+
+
+class Batch:
+    """Many independent solves (own K, p, domain, IC, CFL) stepped together:
+    one persistent launch per stage for all instances, graph-replayed."""
+
+    def __init__(self, cfgs, device="cuda:0", stream=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("fvstuff_amd.dgsem needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.lib = load_library()
+        self.n = len(cfgs)
+        self.cfgs = list(cfgs)
+        self.device = torch.device(device)
+        torch.cuda.set_device(self.device)
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        arr = (DGConfig * self.n)(*cfgs)
+        self._b = C.c_void_p()
+        rc = self.lib.dg_batch_create(arr, self.n, C.c_void_p(self.stream.cuda_stream), C.byref(self._b))
+        if rc != 0:
+            raise DGError(rc, "dg_batch_create failed (mixed switches, AV/limiter requested, or allocation)")
+        self.sizes = [int(self.lib.dg_batch_state_size(self._b, i)) for i in range(self.n)]
+        self.dofs = sum(self.sizes)
+
+    def close(self):
+        if getattr(self, "_b", None) is not None and self._b.value:
+            self.lib.dg_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise DGError(rc, self.lib.dg_batch_last_error(self._b).decode())
+
+    def _ptrs(self, Qs):
+        if len(Qs) != self.n:
+            raise ValueError("one state per instance")
+        return (C.c_void_p * self.n)(*[_ptr(q).value for q in Qs])
+
+    def new_states(self):
+        return [torch.empty(n, dtype=torch.float64, device=self.device) for n in self.sizes]
+
+    def init_states(self, ics, Qs=None):
+        Qs = self.new_states() if Qs is None else Qs
+        arr = (DGICParams * self.n)(*ics)
+        self._check(self.lib.dg_batch_init_state(self._b, arr, self._ptrs(Qs)))
+        return Qs
+
+    def _advance(self, fn, Qs, nsteps, cfl, t_end, t):
+        cfl = [float(cfl)] * self.n if isinstance(cfl, (int, float)) else [float(c) for c in cfl]
+        c = (C.c_double * self.n)(*cfl)
+        tt = (C.c_double * self.n)(*([0.0] * self.n if t is None else t))
+        st = (C.c_int * self.n)()
+        rc = fn(self._b, self._ptrs(Qs), int(nsteps), c, float(t_end), tt, st)
+        self.last_status = list(st)
+        self._check(rc)
+        return list(tt)
+
+    def advance(self, Qs, nsteps, cfl, t_end=0.0, t=None):
+        """nsteps steps of every instance; returns the per-instance times."""
+        return self._advance(self.lib.dg_batch_advance, Qs, nsteps, cfl, t_end, t)
+
+    def advance_each(self, Qs, nsteps, cfl, t_end=0.0, t=None):
+        """The same steps, one dg_advance per instance (for comparison)."""
+        return self._advance(self.lib.dg_batch_advance_each, Qs, nsteps, cfl, t_end, t)
+
+    def set_options(self, use_graph=True, grid_blocks=0):
+        self._check(self.lib.dg_batch_set_options(self._b, int(bool(use_graph)), int(grid_blocks)))
+
+    def autotune(self, Qs, cfl):
+        cfl = [float(cfl)] * self.n if isinstance(cfl, (int, float)) else [float(c) for c in cfl]
+        g = C.c_int()
+        self._check(self.lib.dg_batch_autotune(self._b, self._ptrs(Qs), (C.c_double * self.n)(*cfl), C.byref(g)))
+        return g.value

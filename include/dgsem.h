@@ -167,6 +167,36 @@ int         dg_totals(dg_ctx* ctx, const double* Q, double out[5]);
  * picks a heuristic schedule.  Writes the chosen values if pointers non-NULL. */
 int         dg_autotune(dg_ctx* ctx, int* elems_per_block, int* xcd_remap);
 
+/* ── Batched independent solves (no counterpart in the reference, which runs
+ * one solve per process): ensembles of instances that differ in K, p, domain,
+ * IC and CFL but share the flux / boundary / precision switches.  Every stage
+ * of every instance is one persistent launch over a tile list (heaviest tiles
+ * first); each instance keeps its own device-side dt and time.  The stage
+ * launches of the time loop are replayed from a captured HIP graph.  AV and
+ * the limiter are not available in batched mode (DG_ERR_INVALID_ARG). */
+typedef struct dg_batch dg_batch;
+
+int         dg_batch_create(const dg_config* cfgs, int n, void* stream, dg_batch** out);
+int         dg_batch_destroy(dg_batch* b);
+const char* dg_batch_last_error(const dg_batch* b);
+/* doubles in instance i's state vector */
+long long   dg_batch_state_size(const dg_batch* b, int i);
+/* Qs[i]: device state of instance i (caller-owned), ics[i]: its IC */
+int         dg_batch_init_state(dg_batch* b, const dg_ic_params* ics, double* const* Qs);
+/* nsteps fused SSP-RK3 steps of every instance; cfl[i] per instance; t_inout[i]
+ * in/out (may be NULL); status_out[i] per-instance DG_* code (may be NULL).
+ * Returns the first instance failure. */
+int         dg_batch_advance(dg_batch* b, double* const* Qs, int nsteps, const double* cfl, double t_end,
+                             double* t_inout, int* status_out);
+/* The same steps issued instance by instance with dg_advance (for comparison). */
+int         dg_batch_advance_each(dg_batch* b, double* const* Qs, int nsteps, const double* cfl, double t_end,
+                                  double* t_inout, int* status_out);
+/* Tuning: use_graph (0/1), persistent grid size in workgroups (0 = occupancy x CUs). */
+int         dg_batch_set_options(dg_batch* b, int use_graph, int grid_blocks);
+/* Time a few persistent grid sizes on the given states (they are advanced by
+ * a few steps) and keep the fastest; writes it to *grid_blocks if non-NULL. */
+int         dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* grid_blocks);
+
 #ifdef __cplusplus
 }
 #endif
