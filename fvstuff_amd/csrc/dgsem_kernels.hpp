@@ -44,7 +44,8 @@ struct DevState {
     double dt;                       // dt of the current step
     int status;                      // first numerical failure (ST_*)
     int skip;                        // current step is a no-op (t >= t_end)
-    int pad[2];
+    int steps;                       // steps taken since the last reset
+    int pad;
 };
 
 struct KArgs {
@@ -298,6 +299,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                     go = 1;
                     if (a.stage == 2 && lead) {
                         st->t += d;
+                        st->steps += 1;
                         st->lam_bits[sc.slot_next] = 0ull;
                     }
                 }

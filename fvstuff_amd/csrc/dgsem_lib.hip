@@ -29,6 +29,7 @@ struct dg_ctx {
     double* d_xi = nullptr;              // LGL node coordinates (IC kernel)
     int part_blocks = 0;
     int E = 0, xcd = 0;
+    int last_steps = 0;                  // steps taken by the last dg_advance
     char err[512] = {0};
 };
 
@@ -225,6 +226,7 @@ __global__ void state_reset_kernel(DevState* st, double t0) {
     st->dt = 0.0;
     st->status = ST_OK;
     st->skip = 0;
+    st->steps = 0;
 }
 
 __global__ void slot_reset_kernel(unsigned long long* slot) { *slot = 0ull; }
@@ -344,6 +346,7 @@ __global__ void batch_reset_kernel(const KArgs* args, int n, const double* t0) {
     st->dt = 0.0;
     st->status = ST_OK;
     st->skip = 0;
+    st->steps = 0;
 }
 
 // wave speed of each instance's state, one tile of E elements per visit
@@ -408,6 +411,7 @@ const char* dg_status_string(int s) {
 const char* dg_last_error(const dg_ctx* c) { return c ? c->err : "null context"; }
 
 long long dg_state_size(const dg_ctx* c) { return c ? 4LL * c->NEL * c->N2 : 0; }
+int dg_steps_taken(const dg_ctx* c) { return c ? c->last_steps : 0; }
 long long dg_num_elements(const dg_ctx* c) { return c ? c->NEL : 0; }
 
 int dg_operators(const dg_ctx* c, double* x, double* w, double* D) {
@@ -564,6 +568,7 @@ static int advance_impl(dg_ctx* c, double* Q, int nsteps, double cfl, double t_e
     HIPCK(c, hipStreamSynchronize(c->stream));
     if (t_inout) *t_inout = h.t;
     if (dt_last) *dt_last = h.dt;
+    c->last_steps = h.steps;
     if (h.status != ST_OK) return fail(c, h.status, "%s at t=%.9g", dg_status_string(h.status), h.t);
     if (nsteps > 0 && !h.skip) {   // blow-up check after the last step (:1052-1057)
         double lam;

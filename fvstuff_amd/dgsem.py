@@ -43,6 +43,7 @@ SIGNATURES = {
     "dg_cfl_dt": (C.c_int, [_vp, _vp, C.c_double, _dp, _dp]),
     "dg_advance": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_double, _dp, _dp]),
     "dg_step": (C.c_int, [_vp, _vp, C.c_double, _dp]),
+    "dg_steps_taken": (C.c_int, [_vp]),
     "dg_sensor": (C.c_int, [_vp, _vp, _vp]),
     "dg_totals": (C.c_int, [_vp, _vp, _dp]),
     "dg_autotune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -172,6 +173,18 @@ class Solver:
         self._check(self.lib.dg_advance(self._ctx, _ptr(Q), int(nsteps), float(cfl), float(t_end),
                                         C.byref(tt), C.byref(dt)))
         return tt.value, dt.value
+
+    def steps_taken(self):
+        return int(self.lib.dg_steps_taken(self._ctx))
+
+    def advance_to(self, Q, t_end, cfl, chunk=256, t=0.0, max_steps=10 ** 7):
+        """Advance until t_end in chunks of fused steps (one host sync per
+        chunk); returns (t, steps taken)."""
+        steps = 0
+        while t < t_end - 1e-12 * max(1.0, abs(t_end)) and steps < max_steps:
+            t, _ = self.advance(Q, chunk, cfl, t_end=t_end, t=t)
+            steps += self.steps_taken()
+        return t, steps
 
     def step(self, Q, cfl):
         dt = C.c_double(0.0)

@@ -151,6 +151,10 @@ int         dg_cfl_dt(dg_ctx* ctx, const double* lam_el, double cfl,
 int         dg_advance(dg_ctx* ctx, double* Q, int nsteps, double cfl, double t_end,
                        double* t_inout, double* dt_last);
 
+/* Steps actually taken by the last dg_advance / dg_step on this ctx (steps
+ * past t_end are launched as no-ops and not counted). */
+int         dg_steps_taken(const dg_ctx* ctx);
+
 /* Same as dg_advance with nsteps = 1 (one full step of the reference loop). */
 int         dg_step(dg_ctx* ctx, double* Q, double cfl, double* dt_used);
 
