@@ -13,9 +13,11 @@ namespace dg {
 #define DG_BATCH_LAUNCH(VOLV)                                                                              \
     {                                                                                                    \
         auto k = batch_stage_kernel<DG_REAL, false, VOLV>;                                               \
-        if (lds > 65536) {                                                                               \
+        static size_t done = 0;   /* raised once, never under stream capture */                         \
+        if (lds > 65536 && lds > done) {                                                                 \
             hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
             if (e != hipSuccess) return e;                                                               \
+            done = lds;                                                                                  \
         }                                                                                                \
         hipLaunchKernelGGL(k, dim3(grid), dim3(BLOCK), lds, s, args, tiles, ntiles, stage, sc);          \
         return hipGetLastError();                                                                        \

@@ -58,10 +58,14 @@ __global__ __launch_bounds__(BLOCK) void sensor_kernel(const double* Q, double* 
     }
 }
 
+// Raise a kernel's dynamic-LDS limit once (not per launch: the launches of
+// the fused loop may be under stream capture).  `done` is per instantiation.
 template <typename K>
-static hipError_t set_lds(K kern, size_t lds) {
-    if (lds <= 65536) return hipSuccess;
-    return hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+static hipError_t set_lds(K kern, size_t lds, size_t& done) {
+    if (lds <= 65536 || lds <= done) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess) done = lds;
+    return e;
 }
 
 #define DG_CAT2(a, b) a##b
@@ -69,7 +73,8 @@ static hipError_t set_lds(K kern, size_t lds) {
 
 template <typename Real, bool FULL, int VOL>
 static hipError_t launch_one(const KArgs& a, int grid, int block, size_t lds, hipStream_t s) {
-    hipError_t e = set_lds(stage_kernel<DG_P, Real, FULL, VOL>, lds);
+    static size_t done = 0;
+    hipError_t e = set_lds(stage_kernel<DG_P, Real, FULL, VOL>, lds, done);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((stage_kernel<DG_P, Real, FULL, VOL>), dim3(grid), dim3(block), lds, s, a);
     return hipGetLastError();
@@ -94,7 +99,8 @@ hipError_t DG_CAT(launch_limit_p, DG_P)(double* Q, const double* w64, int NEL, i
                                         double eps_p, hipStream_t s) {
     constexpr int N2 = (DG_P + 1) * (DG_P + 1);
     size_t lds = ((size_t)E * 4 * N2 + 6 * E + 4) * sizeof(double);
-    hipError_t e = set_lds(limit_kernel<DG_P>, lds);
+    static size_t done = 0;
+    hipError_t e = set_lds(limit_kernel<DG_P>, lds, done);
     if (e != hipSuccess) return e;
     int grid = (NEL + E - 1) / E;
     hipLaunchKernelGGL((limit_kernel<DG_P>), dim3(grid), dim3(BLOCK), lds, s, Q, w64, NEL, E, g, eps_rho, eps_p);

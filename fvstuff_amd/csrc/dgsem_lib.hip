@@ -30,6 +30,12 @@ struct dg_ctx {
     int part_blocks = 0;
     int E = 0, xcd = 0;
     int last_steps = 0;                  // steps taken by the last dg_advance
+    // graph replay of the fused loop: two steps per replay, captured on `cap`
+    hipStream_t cap = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    struct { const double* Q; double cfl, t_end; int E, xcd; } gkey = {nullptr, 0, 0, 0, 0};
+    bool use_graph = true;
     char err[512] = {0};
 };
 
@@ -308,7 +314,7 @@ struct StageCall {
     int check_blowup, slot_cur, slot_next;
 };
 
-static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd) {
+static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd, hipStream_t stream) {
     const dg_config& f = c->cfg;
     const bool fp32 = f.precision == DG_MIXED;
     const bool lam = s.mode == MODE_RHS && s.lam_el != nullptr;
@@ -329,7 +335,7 @@ static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd) {
     a.off_U = L.off_U; a.off_av = L.off_av; a.off_lam = L.off_lam; a.off_el = L.off_el;
     const int grid = (c->NEL + E - 1) / E;
     const int block = std::min(BLOCK, (E * 2 * c->N + 63) / 64 * 64);
-    HIPCK(c, stage_launcher(f.p)(a, fp32, full, grid, block, (size_t)L.total, c->stream));
+    HIPCK(c, stage_launcher(f.p)(a, fp32, full, grid, block, (size_t)L.total, stream));
     return DG_OK;
 }
 
@@ -428,6 +434,10 @@ int dg_destroy(dg_ctx* c) {
     if (!c) return DG_OK;
     hipFree(c->d_tab64); hipFree(c->d_tab32); hipFree(c->d_U1); hipFree(c->d_U2);
     hipFree(c->d_st); hipFree(c->d_red); hipFree(c->d_part); hipFree(c->d_xi);
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->cap) hipStreamDestroy(c->cap);
+    if (c->ev_in) hipEventDestroy(c->ev_in);
+    if (c->ev_out) hipEventDestroy(c->ev_out);
     delete c;
     return DG_OK;
 }
@@ -477,6 +487,9 @@ int dg_create(const dg_config* cfg, void* stream, dg_ctx** out) {
     CK_CREATE(hipMemcpyAsync(c->d_xi, c->x, c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CK_CREATE(hipMemcpyAsync(c->d_tab64, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CK_CREATE(hipMemcpyAsync(c->d_tab32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    CK_CREATE(hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking));
+    CK_CREATE(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
+    CK_CREATE(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
     CK_CREATE(hipStreamSynchronize(c->stream));
 #undef CK_CREATE
     *out = c;
@@ -511,7 +524,7 @@ int dg_compute_rhs(dg_ctx* c, const double* Q, double* RHS, double* lam_el) {
     if (!c || !Q || !RHS) return fail(c, DG_ERR_INVALID_ARG, "dg_compute_rhs: null argument");
     StageCall s{};
     s.mode = MODE_RHS; s.stage = 0; s.Up = Q; s.U0 = Q; s.out = RHS; s.lam_el = lam_el;
-    return launch_stage(c, s, c->E, c->xcd);
+    return launch_stage(c, s, c->E, c->xcd, c->stream);
 }
 
 int dg_rk3_stage(dg_ctx* c, int stage, double* Uout, const double* U0, const double* Uprev, const double* L,
@@ -543,26 +556,56 @@ int dg_cfl_dt(dg_ctx* c, const double* lam_el, double cfl, double* lam_max, doub
     return DG_OK;
 }
 
+// Launch the three fused stages of one step; `n` is the step index in the call.
+static int advance_step(dg_ctx* c, double* Q, int n, double cfl, double t_end, int E, int xcd, hipStream_t st) {
+    const int cur = n & 1, nxt = cur ^ 1;
+    StageCall s{};
+    s.mode = MODE_STAGE; s.cfl = cfl; s.t_end = t_end; s.slot_cur = cur; s.slot_next = nxt;
+    s.stage = 1; s.Up = Q; s.U0 = Q; s.out = c->d_U1; s.ca = 0.0; s.cb = 1.0; s.check_blowup = n > 0;
+    int rc = launch_stage(c, s, E, xcd, st);
+    if (rc) return rc;
+    s.stage = 2; s.Up = c->d_U1; s.U0 = Q; s.out = c->d_U2; s.ca = 0.75; s.cb = 0.25;
+    if ((rc = launch_stage(c, s, E, xcd, st))) return rc;
+    s.stage = 3; s.Up = c->d_U2; s.U0 = Q; s.out = Q; s.ca = 1.0 / 3.0; s.cb = 2.0 / 3.0;
+    return launch_stage(c, s, E, xcd, st);
+}
+
 static int advance_impl(dg_ctx* c, double* Q, int nsteps, double cfl, double t_end, double* t_inout,
-                        double* dt_last, int E, int xcd) {
+                        double* dt_last, int E, int xcd, bool graph) {
     const size_t S = (size_t)c->NEL * c->N2;
     const double t0 = t_inout ? *t_inout : 0.0;
     hipLaunchKernelGGL(state_reset_kernel, dim3(1), dim3(1), 0, c->stream, c->d_st, t0);
     hipLaunchKernelGGL(lambda_kernel, dim3(grid_for(S, BLOCK)), dim3(BLOCK), 0, c->stream, Q, S, c->cfg.gamma,
                        &c->d_st->lam_bits[0]);
     HIPCK(c, hipGetLastError());
-    for (int n = 0; n < nsteps; n++) {
-        const int cur = n & 1, nxt = cur ^ 1;
-        StageCall s{};
-        s.mode = MODE_STAGE; s.cfl = cfl; s.t_end = t_end; s.slot_cur = cur; s.slot_next = nxt;
-        s.stage = 1; s.Up = Q; s.U0 = Q; s.out = c->d_U1; s.ca = 0.0; s.cb = 1.0; s.check_blowup = n > 0;
-        int rc = launch_stage(c, s, E, xcd);
-        if (rc) return rc;
-        s.stage = 2; s.Up = c->d_U1; s.U0 = Q; s.out = c->d_U2; s.ca = 0.75; s.cb = 0.25;
-        if ((rc = launch_stage(c, s, E, xcd))) return rc;
-        s.stage = 3; s.Up = c->d_U2; s.U0 = Q; s.out = Q; s.ca = 1.0 / 3.0; s.cb = 2.0 / 3.0;
-        if ((rc = launch_stage(c, s, E, xcd))) return rc;
+    int rc = DG_OK, n = 0;
+    if (nsteps > 0 && (rc = advance_step(c, Q, n++, cfl, t_end, E, xcd, c->stream))) return rc;
+    if (graph && nsteps >= 3) {
+        // steps 1, 2, ... replay a graph of two steps (odd slot parity first)
+        const bool same = c->gexec && c->gkey.Q == Q && c->gkey.cfl == cfl && c->gkey.t_end == t_end &&
+                          c->gkey.E == E && c->gkey.xcd == xcd;
+        if (!same) {
+            if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+            hipGraph_t g;
+            HIPCK(c, hipStreamBeginCapture(c->cap, hipStreamCaptureModeThreadLocal));
+            int r1 = advance_step(c, Q, 1, cfl, t_end, E, xcd, c->cap);
+            int r2 = r1 ? r1 : advance_step(c, Q, 2, cfl, t_end, E, xcd, c->cap);
+            hipError_t ec = hipStreamEndCapture(c->cap, &g);
+            if (r2) return r2;
+            HIPCK(c, ec);
+            hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            HIPCK(c, ei);
+            c->gkey = {Q, cfl, t_end, E, xcd};
+        }
+        HIPCK(c, hipEventRecord(c->ev_in, c->stream));
+        HIPCK(c, hipStreamWaitEvent(c->cap, c->ev_in, 0));
+        for (; n + 1 < nsteps; n += 2) HIPCK(c, hipGraphLaunch(c->gexec, c->cap));
+        HIPCK(c, hipEventRecord(c->ev_out, c->cap));
+        HIPCK(c, hipStreamWaitEvent(c->stream, c->ev_out, 0));
     }
+    for (; n < nsteps; n++)
+        if ((rc = advance_step(c, Q, n, cfl, t_end, E, xcd, c->stream))) return rc;
     DevState h;
     HIPCK(c, hipMemcpyAsync(&h, c->d_st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
@@ -581,7 +624,13 @@ static int advance_impl(dg_ctx* c, double* Q, int nsteps, double cfl, double t_e
 
 int dg_advance(dg_ctx* c, double* Q, int nsteps, double cfl, double t_end, double* t_inout, double* dt_last) {
     if (!c || !Q || nsteps < 0 || !(cfl > 0)) return fail(c, DG_ERR_INVALID_ARG, "dg_advance: bad argument");
-    return advance_impl(c, Q, nsteps, cfl, t_end, t_inout, dt_last, c->E, c->xcd);
+    return advance_impl(c, Q, nsteps, cfl, t_end, t_inout, dt_last, c->E, c->xcd, c->use_graph);
+}
+
+int dg_set_graph(dg_ctx* c, int use_graph) {
+    if (!c) return DG_ERR_INVALID_ARG;
+    c->use_graph = use_graph != 0;
+    return DG_OK;
 }
 
 int dg_step(dg_ctx* c, double* Q, double cfl, double* dt_used) {
@@ -635,10 +684,10 @@ int dg_autotune(dg_ctx* c, int* elems_per_block, int* xcd_remap) {
     for (int x = 0; x < 2 && rc == DG_OK; x++)
         for (int E : Es) {
             double t = 0.0;
-            rc = advance_impl(c, Q, 1, 0.5, 0.0, &t, nullptr, E, x);   // warm-up (LDS attribute, caches)
+            rc = advance_impl(c, Q, 1, 0.5, 0.0, &t, nullptr, E, x, false);   // warm-up (LDS attribute, caches)
             if (rc) break;
             hipEventRecord(e0, c->stream);
-            rc = advance_impl(c, Q, 3, 0.5, 0.0, &t, nullptr, E, x);
+            rc = advance_impl(c, Q, 3, 0.5, 0.0, &t, nullptr, E, x, false);
             hipEventRecord(e1, c->stream);
             hipEventSynchronize(e1);
             if (rc) break;

@@ -44,6 +44,7 @@ SIGNATURES = {
     "dg_advance": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_double, _dp, _dp]),
     "dg_step": (C.c_int, [_vp, _vp, C.c_double, _dp]),
     "dg_steps_taken": (C.c_int, [_vp]),
+    "dg_set_graph": (C.c_int, [_vp, C.c_int]),
     "dg_sensor": (C.c_int, [_vp, _vp, _vp]),
     "dg_totals": (C.c_int, [_vp, _vp, _dp]),
     "dg_autotune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -173,6 +174,9 @@ class Solver:
         self._check(self.lib.dg_advance(self._ctx, _ptr(Q), int(nsteps), float(cfl), float(t_end),
                                         C.byref(tt), C.byref(dt)))
         return tt.value, dt.value
+
+    def set_graph(self, use_graph):
+        self._check(self.lib.dg_set_graph(self._ctx, int(bool(use_graph))))
 
     def steps_taken(self):
         return int(self.lib.dg_steps_taken(self._ctx))

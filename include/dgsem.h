@@ -155,6 +155,11 @@ int         dg_advance(dg_ctx* ctx, double* Q, int nsteps, double cfl, double t_
  * past t_end are launched as no-ops and not counted). */
 int         dg_steps_taken(const dg_ctx* ctx);
 
+/* Replay the fused loop's stage launches from a captured HIP graph (two steps
+ * per replay; default on).  The graph is re-captured when Q, cfl, t_end or
+ * the schedule change.  Results are identical either way. */
+int         dg_set_graph(dg_ctx* ctx, int use_graph);
+
 /* Same as dg_advance with nsteps = 1 (one full step of the reference loop). */
 int         dg_step(dg_ctx* ctx, double* Q, double cfl, double* dt_used);
 

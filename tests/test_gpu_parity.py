@@ -163,8 +163,10 @@ def test_c1_mixed_error(dgsem, c1_oracle):
     print(f"C1 mixed rel L2 {err:.3e}  rel Linf {linf:.3e}")
     assert err <= 1e-6
     assert t == pytest.approx(t_ref, rel=1e-6)
+    # the FP32 right-hand side conserves to FP32 round-off: the rounded D does
+    # not satisfy w_i D_ik = -w_k D_ki exactly (FP64 form: 1e-13, above)
     for q in range(4):
-        assert abs(T1[q] - T0[q]) <= 1e-12 * max(abs(T0[q]), abs(T0[0])), (q, T0[q], T1[q])
+        assert abs(T1[q] - T0[q]) <= 5e-8 * max(abs(T0[q]), abs(T0[0])), (q, T0[q], T1[q])
 
 
 def test_reference_loop_matches_oracle_and_fused(dgsem):
@@ -257,3 +259,19 @@ def test_autotune_keeps_results(dgsem):
     Q = s.init_state(make_ic("vortex"))
     s.advance(Q, 5, 0.5)
     assert np.array_equal(cpu(Q), a)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replay_matches_eager(dgsem, precision):
+    cfg = vortex_config(K=16, p=3, precision=precision)
+    out = []
+    for use_graph in (True, False):
+        s = dgsem.Solver(cfg)
+        s.set_graph(use_graph)
+        Q = s.init_state(make_ic("vortex"))
+        t1, _ = s.advance(Q, 7, 0.5)
+        t2, _ = s.advance(Q, 6, 0.5, t=t1)       # second call replays the cached graph
+        assert s.steps_taken() == 6
+        out.append((t2, cpu(Q)))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
