@@ -82,8 +82,10 @@ def exact_vortex(cfg, t, x):
     K, p = cfg.K, cfg.p
     h = 10.0 / K
     e = np.arange(K, dtype=np.float64)
-    X = h * (e[None, :, None, None] + 0.5 * (x[None, None, None, :] + 1.0)) + 0 * e[:, None, None, None]
-    Y = h * (e[:, None, None, None] + 0.5 * (x[None, None, :, None] + 1.0)) + 0 * e[None, :, None, None]
+    Xb = h * (e[None, :, None, None] + 0.5 * (x[None, None, None, :] + 1.0))    # (1, K, 1, N)
+    Yb = h * (e[:, None, None, None] + 0.5 * (x[None, None, :, None] + 1.0))    # (K, 1, N, 1)
+    X = Xb + 0.0 * Yb
+    Y = Yb + 0.0 * Xb
     g = 1.4; eps = 5.0
     du = np.zeros(X.shape); dv = np.zeros(X.shape); dT = np.zeros(X.shape)
     xs = (X - t) % 10.0; ys = (Y - t) % 10.0
