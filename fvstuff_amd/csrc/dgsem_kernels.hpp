@@ -180,6 +180,15 @@ __device__ __forceinline__ void load_prim64(const double* __restrict__ Q, size_t
     Cons64::to_prim(U, W, g);
 }
 
+// LDS holds node states scaled by 1/2 (exact in binary floating point); the
+// full state is recovered exactly by doubling.
+template <typename Real>
+__device__ __forceinline__ Prim<Real> doubled(const Prim<Real>& h) {
+    Prim<Real> w;
+    w.r = (Real)2 * h.r; w.u = (Real)2 * h.u; w.v = (Real)2 * h.v; w.p = (Real)2 * h.p; w.z = (Real)2 * h.z;
+    return w;
+}
+
 template <typename Real>
 __device__ __forceinline__ void rot(Prim<Real>& w, bool swap) {
     Real u = w.u, v = w.v;
@@ -189,14 +198,14 @@ __device__ __forceinline__ void rot(Prim<Real>& w, bool swap) {
 
 // ── volume contribution of one line: acc[i] = sum_k D_ik F#(U_i,U_k) ──────
 template <int N, typename Real, int VOL>
-__device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Real> (&w)[N], const Real* D,
+__device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Real> (&hw)[N], const Real* D,
                                             Real (&acc)[N][4]) {
     if (VOL == VOL_COLLOCATION) {
         // linear contraction sum_k D_ik f(U_k) (:431-439)
 #pragma unroll
         for (int k = 0; k < N; k++) {
             Real f[4];
-            ph.template flux<0>(w[k], f);
+            ph.template flux<0>(doubled(hw[k]), f);
 #pragma unroll
             for (int i = 0; i < N; i++) {
                 const Real dik = D[i * N + k];
@@ -210,27 +219,45 @@ __device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Rea
 #pragma unroll
     for (int i = 0; i < N; i++) {
         Real f[4];
-        ph.template flux<0>(w[i], f);
+        ph.template flux<0>(doubled(hw[i]), f);
         const Real dii = D[i * N + i];
 #pragma unroll
         for (int q = 0; q < 4; q++) acc[i][q] += dii * f[q];
+    }
+    if (VOL == VOL_RANOCHA) {
+        const Real c[4] = {ph.inv_gm1, ph.inv_gm1 * (Real)(1.0 / 3.0), ph.inv_gm1 * (Real)(1.0 / 5.0),
+                           ph.inv_gm1 * (Real)(1.0 / 7.0)};
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+#pragma unroll
+            for (int k = i + 1; k < N; k++) {
+                Real F[4];
+                Phys<Real>::ranocha_half(hw[i], hw[k], c, ph.inv_gm1, F);
+                const Real dik = D[i * N + k], dki = D[k * N + i];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    acc[i][q] += dik * F[q];
+                    acc[k][q] += dki * F[q];
+                }
+                // one pair in flight at a time: the line's states and
+                // accumulators already fill most of the register budget
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        return;
     }
 #pragma unroll
     for (int i = 0; i < N; i++) {
 #pragma unroll
         for (int k = i + 1; k < N; k++) {
             Real F[4];
-            if (VOL == VOL_RANOCHA) ph.ranocha(w[i], w[k], F);
-            else ph.chandrashekar(w[i], w[k], F);
+            ph.chandrashekar(doubled(hw[i]), doubled(hw[k]), F);
             const Real dik = D[i * N + k], dki = D[k * N + i];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 acc[i][q] += dik * F[q];
                 acc[k][q] += dki * F[q];
             }
-            // keep the scheduler from overlapping many pair fluxes: the line
-            // state and accumulators already hold (p+1)*9 values per lane
-            if (sizeof(Real) == 8 || N > 6) __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
@@ -317,7 +344,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     // LDS regions
     Real* sW = (Real*)smem;                          // [E][N2][5] prims; reused as sR [E][4][N2]
     Real* sR = sW;
-    double* sU = (double*)(smem + a.off_U);         // [E][4][N2] FP64 stage values (limiter)
+    double* sU = (double*)(smem + a.off_U);         // [E][4][N2] FP64 stage input, then stage values
     Real* sUc = (Real*)(smem + a.off_av);           // [E][4][N2] AV: sanitized cons
     Real* sDu = sUc + (size_t)E * 4 * N2;           // [E][N2]
     Real* sDv = sDu + (size_t)E * N2;               // [E][N2]
@@ -332,13 +359,23 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     const bool need_lam = FULL && (a.av || (a.mode == MODE_RHS && a.lam_el != nullptr));
 
     // ── A: load nodes, prims in FP64, store compute-precision prims ──
+    const bool staged = a.mode == MODE_STAGE;
     for (int s = tid; s < E * N2; s += nthr) {
         const int e = s / N2, nd = s - e * N2;
         double W[4] = {1.0, 0.0, 0.0, 1.0};
-        if (e < nE) load_prim64(a.Up, S, (size_t)(el0 + e) * N2 + nd, g, W);
+        if (e < nE) {
+            const size_t o = (size_t)(el0 + e) * N2 + nd;
+            double U[4] = {a.Up[o], a.Up[S + o], a.Up[2 * S + o], a.Up[3 * S + o]};
+            if (staged) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) sU[(size_t)e * 4 * N2 + q * N2 + nd] = U[q];
+            }
+            Cons64::to_prim(U, W, g);
+        }
         Prim<Real> w = make_prim<Real>(W);
         Real* p = sW + ((size_t)e * N2 + nd) * 5;
-        p[0] = w.r; p[1] = w.u; p[2] = w.v; p[3] = w.p; p[4] = w.z;
+        p[0] = (Real)0.5 * w.r; p[1] = (Real)0.5 * w.u; p[2] = (Real)0.5 * w.v;
+        p[3] = (Real)0.5 * w.p; p[4] = (Real)0.5 * w.z;
         if (FULL) {
             if (need_lam) sLam[e * N2 + nd] = Cons64::wave_speed(W, g);
             if (a.av) {
@@ -366,7 +403,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         if (is_line) {
             Real un[N];
 #pragma unroll
-            for (int k = 0; k < N; k++) un[k] = sW[((size_t)le * N2 + lbase + k * lstride) * 5 + (ydir ? 2 : 1)];
+            for (int k = 0; k < N; k++)
+                un[k] = (Real)2 * sW[((size_t)le * N2 + lbase + k * lstride) * 5 + (ydir ? 2 : 1)];
             Real* dst = ydir ? sDv : sDu;
 #pragma unroll
             for (int i = 0; i < N; i++) {
@@ -385,7 +423,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             double lam_e = 0.0;
             for (int nd = 0; nd < N2; nd++) {
                 const Real* p = sW + ((size_t)e * N2 + nd) * 5;
-                Real rho = Mx::fmax_(p[0], (Real)1e-12), c2 = ph.g * p[3] / rho;
+                Real rho = Mx::fmax_((Real)2 * p[0], (Real)1e-12), c2 = ph.g * ((Real)2 * p[3]) / rho;
                 Real divu = Mx::fmin_(jac * (sDu[e * N2 + nd] + sDv[e * N2 + nd]), (Real)0), du2 = divu * divu;
                 Real th = du2 / (du2 + (Real)a.av_k * c2 * dx2inv + (Real)1e-30);
                 th_e = Mx::fmax_(th_e, th);
@@ -404,15 +442,15 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         for (int q = 0; q < 4; q++) acc[i][q] = 0;
 
     if (is_line) {
-        Prim<Real> w[N];
+        Prim<Real> hw[N];    // half-scaled line states
         const Real* eb = sW + (size_t)le * N2 * 5;
 #pragma unroll
         for (int k = 0; k < N; k++) {
             const Real* p = eb + (lbase + k * lstride) * 5;
-            w[k].r = p[0]; w[k].u = p[1]; w[k].v = p[2]; w[k].p = p[3]; w[k].z = p[4];
-            rot(w[k], ydir);
+            hw[k].r = p[0]; hw[k].u = p[1]; hw[k].v = p[2]; hw[k].p = p[3]; hw[k].z = p[4];
+            rot(hw[k], ydir);
         }
-        line_volume<N, Real, VOL>(ph, w, D, acc);
+        line_volume<N, Real, VOL>(ph, hw, D, acc);
         const Real scale = VOL == VOL_COLLOCATION ? -jac : (Real)-2 * jac;
 #pragma unroll
         for (int i = 0; i < N; i++)
@@ -427,7 +465,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             Real cm[4], cp[4];
 #pragma unroll 1
             for (int f = 0; f < 2; f++) {
-                const Prim<Real> own = f ? w[N - 1] : w[0];
+                const Prim<Real> own = doubled(f ? hw[N - 1] : hw[0]);
                 const bool edge = f ? (cpos == a.K - 1) : (cpos == 0);
                 Prim<Real> nb;
                 if (edge && a.bc == BC_WALL) {
@@ -437,9 +475,17 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                     if (edge) c2 = f ? 0 : a.K - 1;
                     const int elnb = ydir ? c2 * a.K + ei : ej * a.K + c2;
                     const int ndnb = lbase + (f ? 0 : N - 1) * lstride;
-                    double W[4];
-                    load_prim64(a.Up, S, (size_t)elnb * N2 + ndnb, g, W);
-                    nb = make_prim<Real>(W);
+                    if (elnb >= el0 && elnb < el0 + nE) {
+                        // neighbour staged by this tile: the same FP64 -> prim
+                        // conversion, so the face sees bit-identical states
+                        const Real* pn = sW + ((size_t)(elnb - el0) * N2 + ndnb) * 5;
+                        nb.r = pn[0]; nb.u = pn[1]; nb.v = pn[2]; nb.p = pn[3]; nb.z = pn[4];
+                        nb = doubled(nb);
+                    } else {
+                        double W[4];
+                        load_prim64(a.Up, S, (size_t)elnb * N2 + ndnb, g, W);
+                        nb = make_prim<Real>(W);
+                    }
                     rot(nb, ydir);
                 }
                 const Prim<Real> L = f ? own : nb, R = f ? nb : own;
@@ -542,14 +588,15 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         const int e = s / N2, nd = s - e * N2;
         const size_t o = (size_t)(el0 + e) * N2 + nd;
         const Real* r = sR + (size_t)e * 4 * N2 + nd;
+        double* up = sU + (size_t)e * 4 * N2 + nd;
         double Us[4];
         if (a.ca != 0.0) {
 #pragma unroll
             for (int q = 0; q < 4; q++)
-                Us[q] = a.ca * a.U0[q * S + o] + a.cb * (a.Up[q * S + o] + dt * (double)r[q * N2]);
+                Us[q] = a.ca * a.U0[q * S + o] + a.cb * (up[q * N2] + dt * (double)r[q * N2]);
         } else {
 #pragma unroll
-            for (int q = 0; q < 4; q++) Us[q] = a.Up[q * S + o] + dt * (double)r[q * N2];
+            for (int q = 0; q < 4; q++) Us[q] = up[q * N2] + dt * (double)r[q * N2];
         }
         Cons64::sanitize_cons(Us, g);
         if (lim) {
@@ -623,12 +670,13 @@ struct LdsLayout {
 
 inline int align_up(int x, int a) { return (x + a - 1) / a * a; }
 
-inline LdsLayout lds_layout(int N, int E, int real_bytes, bool limiter, bool av, bool lam) {
+// `stage`: the FP64 input state is staged in LDS (sU) for the update phase.
+inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, bool lam) {
     const int N2 = N * N;
     LdsLayout L;
     int off = align_up(E * N2 * 5 * real_bytes, 16);
     L.off_U = off;
-    if (limiter) off += align_up(E * 4 * N2 * 8, 16);
+    if (stage) off += align_up(E * 4 * N2 * 8, 16);
     L.off_av = off;
     if (av) off += align_up((E * 4 * N2 + 2 * E * N2 + E) * real_bytes, 16);
     L.off_lam = off;

@@ -320,7 +320,7 @@ static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd, hipStream
     const bool lam = s.mode == MODE_RHS && s.lam_el != nullptr;
     const bool lim = s.mode == MODE_STAGE && f.limiter;
     const bool full = f.av || lim || lam;
-    LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, lim, f.av != 0, lam);
+    LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, s.mode == MODE_STAGE, f.av != 0, lam);
     KArgs a;
     a.Up = s.Up; a.U0 = s.U0; a.out = s.out; a.lam_el = s.lam_el;
     a.tab = fp32 ? c->d_tab32 : c->d_tab64;
@@ -768,7 +768,7 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
         dg_ctx* c = b->ctx[i];
         const dg_config& f = c->cfg;
         const bool fp32 = f.precision == DG_MIXED;
-        LdsLayout L = lds_layout(c->N, max_elems_per_block(c->N), fp32 ? 4 : 8, false, false, false);
+        LdsLayout L = lds_layout(c->N, max_elems_per_block(c->N), fp32 ? 4 : 8, true, false, false);
         for (int s = 1; s <= 3; s++) {
             KArgs a;
             memset(&a, 0, sizeof(a));
@@ -848,7 +848,7 @@ int dg_batch_create(const dg_config* cfgs, int n, void* stream, dg_batch** out) 
         const int nt = (c->NEL + E - 1) / E;
         const double cost = (double)E * c->N * c->N * c->N;
         for (int t = 0; t < nt; t++) tl.push_back({cost, make_int2(i, t)});
-        LdsLayout L = lds_layout(c->N, E, c0.precision == DG_MIXED ? 4 : 8, false, false, false);
+        LdsLayout L = lds_layout(c->N, E, c0.precision == DG_MIXED ? 4 : 8, true, false, false);
         lds = std::max(lds, (size_t)L.total);
     }
     std::stable_sort(tl.begin(), tl.end(), [](const std::pair<double, int2>& x, const std::pair<double, int2>& y) {

@@ -146,6 +146,41 @@ template <typename Real> struct Phys {
         F[3] = f1 * eint + (Real)0.5 * (L.p * R.u + R.p * L.u);
     }
 
+    // Ranocha flux on half-scaled node states h = (rho, u, v, p, rho/p)/2, the
+    // form the line loop uses: (a+b)/2 = aL/2 + aR/2 needs no multiply, one
+    // reciprocal of the product (rho_L+rho_R)(z_L+z_R) serves both log means,
+    // and c[] = 1/(g-1) * (1, 1/3, 1/5, 1/7) folds the energy factor into the
+    // series of the inverse log mean of z = rho/p.
+    __device__ __forceinline__ static void ranocha_half(const Prim<Real>& L, const Prim<Real>& R, const Real c[4],
+                                                        Real inv_gm1, Real F[4]) {
+        const Real sr = L.r + R.r, sz = L.z + R.z;          // (rho_L+rho_R)/2, (z_L+z_R)/2
+        const Real rr = Mx::rcp(sr * sz);
+        const Real ir = sz * rr, iz = sr * rr;              // 2/(rho_L+rho_R), 2/(z_L+z_R)
+        const Real dr = L.r - R.r, dz = L.z - R.z;
+        const Real fr = dr * ir, ur = fr * fr;
+        const Real fz = dz * iz, uz = fz * fz;
+        Real r_ln = sr * ((Real)1 - ur * ((Real)(1.0 / 3.0) + ur * ((Real)(4.0 / 45.0) + ur * (Real)(44.0 / 945.0))));
+        Real izg = iz * (c[0] + uz * (c[1] + uz * (c[2] + uz * c[3])));    // 1/((g-1) z_ln)
+        // ratio-form log where the series does not apply; the branch is
+        // wave-uniform (a divergent one would make the register allocator keep
+        // both sides' values of every unrolled pair live)
+        const bool lr = ur >= (Real)Mx::series_thr, lz = uz >= (Real)Mx::series_thr;
+        if (__any(lr || lz)) {
+            const Real r_ex = Mx::div(dr, Mx::ln(Mx::div(L.r, R.r)));
+            const Real z_ex = inv_gm1 * Mx::div(Mx::ln(Mx::div(L.z, R.z)), (Real)2 * dz);
+            r_ln = lr ? r_ex : r_ln;
+            izg = lz ? z_ex : izg;
+        }
+        const Real u_av = L.u + R.u, v_av = L.v + R.v, p_av = L.p + R.p;
+        const Real f1 = r_ln * u_av;
+        const Real ka = L.u * R.u + L.v * R.v;              // (uL uR + vL vR)/4
+        const Real kb = L.p * R.u + R.p * L.u;              // (pL uR + pR uL)/4
+        F[0] = f1;
+        F[1] = f1 * u_av + p_av;
+        F[2] = f1 * v_av;
+        F[3] = (Real)2 * (f1 * ka + kb) + f1 * izg;
+    }
+
     // Chandrashekar EC flux (:259-285) along x, beta = rho/(2p) = z/2.
     __device__ __forceinline__ void chandrashekar(const Prim<Real>& L, const Prim<Real>& R, Real F[4]) const {
         Real bL = (Real)0.5 * L.z, bR = (Real)0.5 * R.z;
