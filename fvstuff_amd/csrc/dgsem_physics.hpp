@@ -166,7 +166,7 @@ template <typename Real> struct Phys {
         // both sides' values of every unrolled pair live)
         const bool lr = ur >= (Real)Mx::series_thr, lz = uz >= (Real)Mx::series_thr;
         if (__any(lr || lz)) {
-            const Real r_ex = Mx::div(dr, Mx::ln(Mx::div(L.r, R.r)));
+            const Real r_ex = Mx::div((Real)2 * dr, Mx::ln(Mx::div(L.r, R.r)));   // dr is (rho_L-rho_R)/2
             const Real z_ex = inv_gm1 * Mx::div(Mx::ln(Mx::div(L.z, R.z)), (Real)2 * dz);
             r_ln = lr ? r_ex : r_ln;
             izg = lz ? z_ex : izg;

@@ -275,3 +275,21 @@ def test_graph_replay_matches_eager(dgsem, precision):
         out.append((t2, cpu(Q)))
     assert out[0][0] == out[1][0]
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("vol,surf", [("ranocha", "rusanov"), ("chandrashekar", "hllc")])
+def test_mixed_rhs_on_discontinuous_data(dgsem, vol, surf):
+    """Sod and KH states exercise the log branch of the log means (pairs with
+    (a-b)/(a+b) beyond the series range) in both precisions."""
+    for cfg, ic in [(make_config(K=10, p=3, volume_flux=vol, surface_flux=surf, av=False), "sod"),
+                    (kh_config(K=8, p=5, volume_flux=vol, surface_flux=surf, precision="fp64"), "kh")]:
+        o = Oracle(cfg)
+        Q = o.init_state(make_ic(ic, noise_amp=0.3) if ic == "kh" else make_ic(ic))
+        R_ref, _ = o.compute_rhs(Q)
+        for prec, tol in (("fp64", 1e-13), ("mixed", 2e-6)):
+            cfg.precision = 0 if prec == "fp64" else 1
+            s = dgsem.Solver(cfg)
+            R, _ = s.compute_rhs(gpu(Q))
+            err = rel_l2(cpu(R), R_ref, w2(cfg))
+            print(f"{vol}/{surf} {ic} {prec}: rel L2 {err:.2e}")
+            assert err < tol, (prec, err)
