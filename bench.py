@@ -154,7 +154,14 @@ def bench_kh(args):
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        t, steps = s.advance_to(Q, args.t_end, args.cfl, chunk=512)
+        last = [time.perf_counter()]
+
+        def progress(t, steps):
+            if time.perf_counter() - last[0] > 10.0:
+                last[0] = time.perf_counter()
+                log(f"[bench-kh] {form}: t={t:.4f} steps={steps}")
+
+        t, steps = s.advance_to(Q, args.t_end, args.cfl, chunk=512, progress=progress)
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1)

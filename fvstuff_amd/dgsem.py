@@ -181,13 +181,16 @@ class Solver:
     def steps_taken(self):
         return int(self.lib.dg_steps_taken(self._ctx))
 
-    def advance_to(self, Q, t_end, cfl, chunk=256, t=0.0, max_steps=10 ** 7):
+    def advance_to(self, Q, t_end, cfl, chunk=256, t=0.0, max_steps=10 ** 7, progress=None):
         """Advance until t_end in chunks of fused steps (one host sync per
-        chunk); returns (t, steps taken)."""
+        chunk); returns (t, steps taken).  progress(t, steps) is called after
+        every chunk if given."""
         steps = 0
         while t < t_end - 1e-12 * max(1.0, abs(t_end)) and steps < max_steps:
             t, _ = self.advance(Q, chunk, cfl, t_end=t_end, t=t)
             steps += self.steps_taken()
+            if progress is not None:
+                progress(t, steps)
         return t, steps
 
     def step(self, Q, cfl):
