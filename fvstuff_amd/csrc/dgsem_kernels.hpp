@@ -216,9 +216,11 @@ __device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Rea
         }
         return;
     }
-    // diagonal: F#(U_i,U_i) = f(U_i) (consistency)
+    // diagonal: F#(U_i,U_i) = f(U_i) (consistency).  On LGL nodes D_ii = 0
+    // except D_00 = -p(p+1)/4 and D_pp = p(p+1)/4 (the negative-sum
+    // construction leaves ~1e-16 on the interior diagonal, dropped here).
 #pragma unroll
-    for (int i = 0; i < N; i++) {
+    for (int i = 0; i < N; i += N - 1) {
         Real f[4];
         ph.template flux<0>(doubled(hw[i]), f);
         const Real dii = D[i * N + i];

@@ -50,33 +50,45 @@ template <typename Real> struct Prim {
 
 // ── FP64 state-side helpers (sanitize + prim conversion), :208-229, :347-355 ──
 struct Cons64 {
+    // 1/x for x in the normal range used here ([1e-14, 1e30]): the hardware
+    // reciprocal estimate plus two Newton steps (within ~1 ulp of 1/x; no
+    // denormal / overflow scaling, which the IEEE division sequence adds).
+    static __device__ __forceinline__ double rcp(double x) {
+        double y = __builtin_amdgcn_rcp(x);
+        double e = fma(-x, y, 1.0);
+        y = fma(y, e, y);
+        e = fma(-x, y, 1.0);
+        return fma(y, e, y);
+    }
     static __device__ __forceinline__ double pressure_from_cons(const double U[4], double g) {
         double rho = fmax(U[0], 1e-14);
-        double ke = 0.5 * (U[1] * U[1] + U[2] * U[2]) / rho;
+        double ke = 0.5 * (U[1] * U[1] + U[2] * U[2]) * rcp(rho);
         return (g - 1.0) * (U[3] - ke);
     }
     static __device__ __forceinline__ void sanitize_cons(double U[4], double g) {
         U[0] = fmax(U[0], 1e-14);
-        double p = pressure_from_cons(U, g);
-        if (p < 1e-14) U[3] = 1e-14 / (g - 1.0) + 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0];
+        double ke = 0.5 * (U[1] * U[1] + U[2] * U[2]) * rcp(U[0]);
+        if ((g - 1.0) * (U[3] - ke) < 1e-14) U[3] = 1e-14 / (g - 1.0) + ke;
     }
     // cons_to_prim_sane: sanitize, convert, floor rho/p, clamp |u|,|v| <= 1e4
-    static __device__ __forceinline__ void to_prim(const double Uin[4], double W[4], double g) {
-        double U[4] = {Uin[0], Uin[1], Uin[2], Uin[3]};
-        sanitize_cons(U, g);
-        W[0] = U[0];
-        W[1] = U[1] / U[0];
-        W[2] = U[2] / U[0];
-        W[3] = (g - 1.0) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0]);
-        W[0] = fmax(W[0], 1e-14);
-        W[3] = fmax(W[3], 1e-14);
+    // (one reciprocal of rho serves u, v and the kinetic energy)
+    static __device__ __forceinline__ void to_prim(const double U[4], double W[4], double g) {
+        const double r = fmax(U[0], 1e-14);
+        const double ri = rcp(r);
+        const double ke = 0.5 * (U[1] * U[1] + U[2] * U[2]) * ri;
+        double E = U[3];
+        if ((g - 1.0) * (E - ke) < 1e-14) E = 1e-14 / (g - 1.0) + ke;
+        W[0] = r;
+        W[1] = U[1] * ri;
+        W[2] = U[2] * ri;
+        W[3] = fmax((g - 1.0) * (E - ke), 1e-14);
         if (W[0] < 1e-5) { W[1] = 0.0; W[2] = 0.0; return; }
         W[1] = fmax(fmin(W[1], 1e4), -1e4);
         W[2] = fmax(fmin(W[2], 1e4), -1e4);
     }
     // max wave speed |u|+|v|+c of sanitized primitives (:483-486)
     static __device__ __forceinline__ double wave_speed(const double W[4], double g) {
-        double rho = fmax(W[0], 1e-12), c2 = g * W[3] / rho;
+        double c2 = g * W[3] * rcp(fmax(W[0], 1e-12));
         return fabs(W[1]) + fabs(W[2]) + sqrt(fmax(c2, 1e-14));
     }
 };
@@ -85,7 +97,8 @@ template <typename Real>
 __device__ __forceinline__ Prim<Real> make_prim(const double W[4]) {
     Prim<Real> w;
     w.r = (Real)W[0]; w.u = (Real)W[1]; w.v = (Real)W[2]; w.p = (Real)W[3];
-    w.z = (Real)(W[0] / W[3]);
+    if (sizeof(Real) == 8) w.z = (Real)(W[0] * Cons64::rcp(W[3]));
+    else w.z = w.r * M<Real>::rcp(w.p);     // FP32 form: rho/p in FP32
     return w;
 }
 
