@@ -61,6 +61,8 @@ struct KArgs {
     int check_blowup;    // stage 1: the wave speed in slot_cur came from a step of this call
     double h, gamma, ca, cb, cfl, t_end, av_cav, av_k, eps_rho, eps_p;
     int off_U, off_av, off_lam, off_el;   // byte offsets of the LDS regions
+    int off_U0;          // LDS copy of U^n (stages 2, 3) when u0_lds
+    int u0_lds;          // prefetch U^n into LDS during the load phase
 };
 
 // Per-launch step control of the fused stepping (ping-pong wave-speed slots).
@@ -296,8 +298,12 @@ __device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Rea
 // live ((p+1)*9 values per lane, twice the registers in FP64).  The floor on
 // waves per SIMD lets FP64 at p >= 5 use the full 512-register file (VGPRs +
 // AGPRs) instead of spilling to scratch.
-template <int P, typename Real> struct Occ { static constexpr int waves = 3; };
-template <int P> struct Occ<P, double> { static constexpr int waves = P >= 5 ? 1 : 2; };
+template <int P, typename Real, int VOL, bool FULL> struct Occ { static constexpr int waves = 3; };
+template <int P, int VOL, bool FULL> struct Occ<P, double, VOL, FULL> {
+    // at p >= 5 only the lean Ranocha kernel fits in 256 registers; the
+    // others get the whole file rather than spill
+    static constexpr int waves = (P >= 5 && (VOL != VOL_RANOCHA || FULL)) ? 1 : 2;
+};
 
 // One tile (E consecutive elements) of one stage.  `lead` marks the one tile
 // per launch that updates the device step state.  Every early exit is taken
@@ -389,18 +395,46 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     const bool need_lam = FULL && (a.av || (a.mode == MODE_RHS && a.lam_el != nullptr));
 
     // ── A: load nodes, prims in FP64, store compute-precision prims ──
+    // A thread owns at most ceil((p+1)/2) node slots (E(p+1)^2 nodes over at
+    // least 2E(p+1) threads).  All of its global loads - the stage input and,
+    // for stages 2 and 3, U^n needed only by the update - are issued before
+    // any conversion, so each wave keeps many requests in flight.
+    constexpr int JMAX = (N + 1) / 2;
     const bool staged = a.mode == MODE_STAGE;
-    for (int s = tid; s < E * N2; s += nthr) {
+    // (FP32 form only: the FP64 form has no registers to spare for it)
+    const bool pre0 = sizeof(Real) == 4 && staged && a.u0_lds && a.ca != 0.0;
+    double* sU0 = (double*)(smem + a.off_U0);       // [E][4][N2] U^n copy
+    double Ul[JMAX][4], U0l[JMAX][4];
+#pragma unroll
+    for (int j = 0; j < JMAX; j++) {
+        const int s = tid + j * nthr;
+        const int e = s / N2, nd = s - e * N2;
+        if (s < E * N2 && e < nE) {
+            const size_t o = (size_t)(el0 + e) * N2 + nd;
+#pragma unroll
+            for (int q = 0; q < 4; q++) Ul[j][q] = a.Up[q * S + o];
+            if (pre0) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) U0l[j][q] = a.U0[q * S + o];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < JMAX; j++) {
+        const int s = tid + j * nthr;
+        if (s >= E * N2) continue;
         const int e = s / N2, nd = s - e * N2;
         double W[4] = {1.0, 0.0, 0.0, 1.0};
         if (e < nE) {
-            const size_t o = (size_t)(el0 + e) * N2 + nd;
-            double U[4] = {a.Up[o], a.Up[S + o], a.Up[2 * S + o], a.Up[3 * S + o]};
             if (staged) {
 #pragma unroll
-                for (int q = 0; q < 4; q++) sU[(size_t)e * 4 * N2 + q * N2 + nd] = U[q];
+                for (int q = 0; q < 4; q++) sU[(size_t)e * 4 * N2 + q * N2 + nd] = Ul[j][q];
             }
-            Cons64::to_prim(U, W, g);
+            if (pre0) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) sU0[(size_t)e * 4 * N2 + q * N2 + nd] = U0l[j][q];
+            }
+            Cons64::to_prim(Ul[j], W, g);
         }
         Prim<Real> w = make_prim<Real>(W);
         Real* p = sW + ((size_t)e * N2 + nd) * 5;
@@ -620,7 +654,11 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         const Real* r = sR + (size_t)e * 4 * N2 + nd;
         double* up = sU + (size_t)e * 4 * N2 + nd;
         double Us[4];
-        if (a.ca != 0.0) {
+        if (pre0) {
+            const double* u0 = sU0 + (size_t)e * 4 * N2 + nd;
+#pragma unroll
+            for (int q = 0; q < 4; q++) Us[q] = a.ca * u0[q * N2] + a.cb * (up[q * N2] + dt * (double)r[q * N2]);
+        } else if (a.ca != 0.0) {
 #pragma unroll
             for (int q = 0; q < 4; q++)
                 Us[q] = a.ca * a.U0[q * S + o] + a.cb * (up[q * N2] + dt * (double)r[q * N2]);
@@ -661,7 +699,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
 }
 
 template <int P, typename Real, bool FULL, int VOL>
-__global__ __launch_bounds__(BLOCK, (Occ<P, Real>::waves)) void stage_kernel(KArgs a) {
+__global__ __launch_bounds__(BLOCK, (Occ<P, Real, VOL, FULL>::waves)) void stage_kernel(KArgs a) {
     const StepCtl sc{a.slot_cur, a.slot_next, a.check_blowup};
     stage_tile<P, Real, FULL, VOL>(a, sc, tile_of_block(blockIdx.x, gridDim.x, a.xcd), blockIdx.x == 0);
 }
@@ -695,13 +733,14 @@ __global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_ke
 
 // Host-side LDS layout shared by the launcher and the kernel.
 struct LdsLayout {
-    int off_U, off_av, off_lam, off_el, total;
+    int off_U, off_av, off_lam, off_el, off_U0, total;
 };
 
 inline int align_up(int x, int a) { return (x + a - 1) / a * a; }
 
-// `stage`: the FP64 input state is staged in LDS (sU) for the update phase.
-inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, bool lam) {
+// `stage`: the FP64 input state is staged in LDS (sU) for the update phase;
+// `u0`: U^n is prefetched into LDS too.
+inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, bool lam, bool u0 = false) {
     const int N2 = N * N;
     LdsLayout L;
     int off = align_up(E * N2 * 5 * real_bytes, 16);
@@ -713,6 +752,8 @@ inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, b
     if (lam || av) off += align_up(E * N2 * 8, 16);
     L.off_el = off;
     off += align_up((6 * E + BLOCK / 64) * 8, 16);
+    L.off_U0 = off;
+    if (u0) off += align_up(E * 4 * N2 * 8, 16);
     L.total = off;
     return L;
 }

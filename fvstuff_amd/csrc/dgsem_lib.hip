@@ -29,6 +29,7 @@ struct dg_ctx {
     double* d_xi = nullptr;              // LGL node coordinates (IC kernel)
     int part_blocks = 0;
     int E = 0, xcd = 0;
+    int u0_lds = 0;                      // prefetch U^n into LDS (tuned)
     int last_steps = 0;                  // steps taken by the last dg_advance
     // graph replay of the fused loop: two steps per replay, captured on `cap`
     hipStream_t cap = nullptr;
@@ -320,8 +321,10 @@ static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd, hipStream
     const bool lam = s.mode == MODE_RHS && s.lam_el != nullptr;
     const bool lim = s.mode == MODE_STAGE && f.limiter;
     const bool full = f.av || lim || lam;
-    LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, s.mode == MODE_STAGE, f.av != 0, lam);
+    const bool u0 = s.mode == MODE_STAGE && c->u0_lds;
+    LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, s.mode == MODE_STAGE, f.av != 0, lam, u0);
     KArgs a;
+    memset(&a, 0, sizeof(a));
     a.Up = s.Up; a.U0 = s.U0; a.out = s.out; a.lam_el = s.lam_el;
     a.tab = fp32 ? c->d_tab32 : c->d_tab64;
     a.w64 = (const double*)c->d_tab64 + c->N2;
@@ -333,6 +336,7 @@ static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd, hipStream
     a.h = c->h; a.gamma = f.gamma; a.ca = s.ca; a.cb = s.cb; a.cfl = s.cfl; a.t_end = s.t_end;
     a.av_cav = f.av_cav; a.av_k = f.av_ksensor; a.eps_rho = f.eps_rho; a.eps_p = f.eps_p;
     a.off_U = L.off_U; a.off_av = L.off_av; a.off_lam = L.off_lam; a.off_el = L.off_el;
+    a.off_U0 = L.off_U0; a.u0_lds = u0;
     const int grid = (c->NEL + E - 1) / E;
     const int block = std::min(BLOCK, (E * 2 * c->N + 63) / 64 * 64);
     HIPCK(c, stage_launcher(f.p)(a, fp32, full, grid, block, (size_t)L.total, stream));
@@ -680,11 +684,13 @@ int dg_autotune(dg_ctx* c, int* elems_per_block, int* xcd_remap) {
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     float best = 1e30f;
-    int bestE = c->E, bestX = c->xcd;
+    int bestE = c->E, bestX = c->xcd, bestU = c->u0_lds;
     // every candidate starts from the same smooth state and takes a few small
     // steps (CFL 0.1 is stable for every degree), so numerics never interfere
-    for (int x = 0; x < 2 && rc == DG_OK; x++)
+    for (int cand = 0; cand < 4 && rc == DG_OK; cand++)
         for (int E : Es) {
+            const int x = cand & 1;
+            c->u0_lds = cand >> 1;
             double t = 0.0;
             if ((rc = dg_init_state(c, &ic, Q))) break;
             rc = advance_impl(c, Q, 1, 0.1, 0.0, &t, nullptr, E, x, false);   // warm-up (LDS attribute, caches)
@@ -696,7 +702,7 @@ int dg_autotune(dg_ctx* c, int* elems_per_block, int* xcd_remap) {
             if (rc) break;
             float ms = 0.f;
             hipEventElapsedTime(&ms, e0, e1);
-            if (ms < best) { best = ms; bestE = E; bestX = x; }
+            if (ms < best) { best = ms; bestE = E; bestX = x; bestU = c->u0_lds; }
         }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -704,6 +710,7 @@ int dg_autotune(dg_ctx* c, int* elems_per_block, int* xcd_remap) {
     if (rc) return rc;
     c->E = bestE;
     c->xcd = bestX;
+    c->u0_lds = bestU;
     if (elems_per_block) *elems_per_block = bestE;
     if (xcd_remap) *xcd_remap = bestX;
     return DG_OK;
@@ -788,6 +795,7 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
             a.cfl = cfl[i]; a.t_end = t_end;
             a.av_cav = f.av_cav; a.av_k = f.av_ksensor; a.eps_rho = f.eps_rho; a.eps_p = f.eps_p;
             a.off_U = L.off_U; a.off_av = L.off_av; a.off_lam = L.off_lam; a.off_el = L.off_el;
+            a.off_U0 = L.off_U0; a.u0_lds = 0;
             b->h_args[(size_t)i * 3 + (s - 1)] = a;
         }
     }
