@@ -232,49 +232,19 @@ __device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Rea
     if (VOL == VOL_RANOCHA) {
         const Real c[4] = {ph.inv_gm1, ph.inv_gm1 * (Real)(1.0 / 3.0), ph.inv_gm1 * (Real)(1.0 / 5.0),
                            ph.inv_gm1 * (Real)(1.0 / 7.0)};
-        // Is every pair of this line in the log-mean series range?  For
-        // a >= b > 0, (a-b)/(a+b) grows with a and falls with b, so the line's
-        // extreme values bound every pair.  Decided per wave (uniform branch).
-        Real rmin = hw[0].r, rmax = hw[0].r, zmin = hw[0].z, zmax = hw[0].z;
 #pragma unroll
-        for (int k = 1; k < N; k++) {
-            rmin = Mx::fmin_(rmin, hw[k].r); rmax = Mx::fmax_(rmax, hw[k].r);
-            zmin = Mx::fmin_(zmin, hw[k].z); zmax = Mx::fmax_(zmax, hw[k].z);
-        }
-        const Real fr = (rmax - rmin) * Mx::rcp(rmax + rmin), fz = (zmax - zmin) * Mx::rcp(zmax + zmin);
-        // a small margin keeps the bound conservative under rounding
-        const bool smooth = fr * fr < (Real)0.99 * (Real)Mx::series_thr && fz * fz < (Real)0.99 * (Real)Mx::series_thr;
-        if (__all(smooth)) {
+        for (int i = 0; i < N; i++) {
 #pragma unroll
-            for (int i = 0; i < N; i++)
+            for (int k = i + 1; k < N; k++) {
+                Real F[4];
+                Phys<Real>::ranocha_half(hw[i], hw[k], c, ph.inv_gm1, F);
+                const Real dik = D[i * N + k], dki = D[k * N + i];
 #pragma unroll
-                for (int k = i + 1; k < N; k++) {
-                    Real F[4];
-                    Phys<Real>::template ranocha_half<false>(hw[i], hw[k], c, ph.inv_gm1, 0, 0, 0, 0, F);
-                    const Real dik = D[i * N + k], dki = D[k * N + i];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        acc[i][q] += dik * F[q];
-                        acc[k][q] += dki * F[q];
-                    }
+                for (int q = 0; q < 4; q++) {
+                    acc[i][q] += dik * F[q];
+                    acc[k][q] += dki * F[q];
                 }
-        } else {
-            Real lr[N], lz[N];
-#pragma unroll
-            for (int k = 0; k < N; k++) { lr[k] = Mx::ln(hw[k].r); lz[k] = Mx::ln(hw[k].z); }
-#pragma unroll
-            for (int i = 0; i < N; i++)
-#pragma unroll
-                for (int k = i + 1; k < N; k++) {
-                    Real F[4];
-                    Phys<Real>::template ranocha_half<true>(hw[i], hw[k], c, ph.inv_gm1, lr[i], lr[k], lz[i], lz[k], F);
-                    const Real dik = D[i * N + k], dki = D[k * N + i];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        acc[i][q] += dik * F[q];
-                        acc[k][q] += dki * F[q];
-                    }
-                }
+            }
         }
         return;
     }
@@ -300,9 +270,9 @@ __device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Rea
 // AGPRs) instead of spilling to scratch.
 template <int P, typename Real, int VOL, bool FULL> struct Occ { static constexpr int waves = 3; };
 template <int P, int VOL, bool FULL> struct Occ<P, double, VOL, FULL> {
-    // at p >= 5 only the lean Ranocha kernel fits in 256 registers; the
-    // others get the whole file rather than spill
-    static constexpr int waves = (P >= 5 && (VOL != VOL_RANOCHA || FULL)) ? 1 : 2;
+    // at p >= 5 the FP64 line state and accumulators plus one pair's
+    // temporaries exceed 256 registers: use the whole file rather than spill
+    static constexpr int waves = P >= 5 ? 1 : 2;
 };
 
 // One tile (E consecutive elements) of one stage.  `lead` marks the one tile

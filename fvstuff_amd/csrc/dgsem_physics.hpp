@@ -164,15 +164,8 @@ template <typename Real> struct Phys {
     // reciprocal of the product (rho_L+rho_R)(z_L+z_R) serves both log means,
     // and c[] = 1/(g-1) * (1, 1/3, 1/5, 1/7) folds the energy factor into the
     // series of the inverse log mean of z = rho/p.
-    // GENERAL = false: every pair is in the series range (the caller checked
-    // the line's min/max ratio); no branch, no select.
-    // GENERAL = true: pairs outside the series range use the log form with
-    // the per-node logarithms lnL, lnR (ln a_L - ln a_R = ln(a_L/a_R)),
-    // selected per lane without branching.
-    template <bool GENERAL>
     __device__ __forceinline__ static void ranocha_half(const Prim<Real>& L, const Prim<Real>& R, const Real c[4],
-                                                        Real inv_gm1, Real lrL, Real lrR, Real lzL, Real lzR,
-                                                        Real F[4]) {
+                                                        Real inv_gm1, Real F[4]) {
         const Real sr = L.r + R.r, sz = L.z + R.z;          // (rho_L+rho_R)/2, (z_L+z_R)/2
         const Real rr = Mx::rcp(sr * sz);
         const Real ir = sz * rr, iz = sr * rr;              // 2/(rho_L+rho_R), 2/(z_L+z_R)
@@ -181,11 +174,15 @@ template <typename Real> struct Phys {
         const Real fz = dz * iz, uz = fz * fz;
         Real r_ln = sr * ((Real)1 - ur * ((Real)(1.0 / 3.0) + ur * ((Real)(4.0 / 45.0) + ur * (Real)(44.0 / 945.0))));
         Real izg = iz * (c[0] + uz * (c[1] + uz * (c[2] + uz * c[3])));    // 1/((g-1) z_ln)
-        if (GENERAL) {
-            const bool lr = ur >= (Real)Mx::series_thr, lz = uz >= (Real)Mx::series_thr;
-            const Real gr = lrL - lrR, gz = lzL - lzR;          // ln(a_L/a_R); 0 only in the series range
-            const Real r_ex = (Real)2 * dr * Mx::rcp(lr ? gr : (Real)1);     // dr is (rho_L-rho_R)/2
-            const Real z_ex = inv_gm1 * gz * Mx::rcp(lz ? (Real)2 * dz : (Real)1);
+        // ratio-form log where the series does not apply.  The branch is
+        // wave-uniform (a divergent one would make the register allocator keep
+        // both sides' values of every unrolled pair live) and each lane then
+        // selects its own value, so results do not depend on how lines are
+        // grouped into waves.
+        const bool lr = ur >= (Real)Mx::series_thr, lz = uz >= (Real)Mx::series_thr;
+        if (__any(lr || lz)) {
+            const Real r_ex = Mx::div((Real)2 * dr, Mx::ln(Mx::div(L.r, R.r)));   // dr is (rho_L-rho_R)/2
+            const Real z_ex = inv_gm1 * Mx::div(Mx::ln(Mx::div(L.z, R.z)), (Real)2 * dz);
             r_ln = lr ? r_ex : r_ln;
             izg = lz ? z_ex : izg;
         }
