@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the error report")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--schedule", default="",
+                    help="E,X,U: elements per workgroup, XCD remap, U^n prefetch (implies --no-autotune)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--workload", default="vortex", choices=["vortex", "kh", "batched", "copy"],
                     help="vortex: the headline line; kh: Kelvin-Helmholtz (--K, --p, --t-end); "
@@ -252,9 +254,14 @@ def main():
     dof = 4 * K * K * N * N
     res = {}
     sols = {}
+    sched = {}
+    if args.schedule:
+        E_, X_, U_ = (int(v) for v in args.schedule.split(","))
+        sched = dict(elems_per_block=E_, xcd_remap=X_, u0_prefetch=U_)
     for form in args.forms.split(","):
-        cfg = vortex_config(K=K, p=p, precision=form)
-        s, Q, ms, t, tuned = time_form(D, torch, cfg, ic, args.steps, args.warmup, args.cfl, not args.no_autotune)
+        cfg = vortex_config(K=K, p=p, precision=form, **sched)
+        s, Q, ms, t, tuned = time_form(D, torch, cfg, ic, args.steps, args.warmup, args.cfl,
+                                       not (args.no_autotune or args.schedule))
         rate = dof * 3 * args.steps / (ms * 1e-3)
         res[form] = dict(ms=ms, rate=rate, t=t, tuned=tuned)
         sols[form] = (s, Q, cfg)

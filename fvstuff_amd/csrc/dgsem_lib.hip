@@ -394,7 +394,7 @@ void dg_config_default(dg_config* f) {
     f->volume_flux = DG_VOL_CHANDRASHEKAR; f->surface_flux = DG_SURF_HLLC; f->bc = DG_BC_WALL;
     f->av = 1; f->limiter = 1; f->precision = DG_FP64;
     f->av_cav = 0.25; f->av_ksensor = 0.1; f->eps_rho = 1e-6; f->eps_p = 1e-6;
-    f->elems_per_block = 0; f->xcd_remap = -1;
+    f->elems_per_block = 0; f->xcd_remap = -1; f->u0_prefetch = -1;
 }
 
 void dg_ic_default(dg_ic_params* ic, int kind) {
@@ -465,6 +465,7 @@ int dg_create(const dg_config* cfg, void* stream, dg_ctx** out) {
     const int Emax = max_elems_per_block(c->N);
     c->E = cfg->elems_per_block > 0 ? std::min(cfg->elems_per_block, Emax) : default_E(c->N, c->NEL);
     c->xcd = cfg->xcd_remap >= 0 ? (cfg->xcd_remap != 0) : 1;
+    c->u0_lds = cfg->u0_prefetch > 0;
 
     // tables: D[N2], w[N], winv[N]
     std::vector<double> t64(c->N2 + 2 * c->N);

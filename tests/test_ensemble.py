@@ -1,0 +1,25 @@
+"""The seed-drawn ensemble of vortex instances (batched workload): ranges,
+degrees, determinism and the per-degree stable CFL."""
+import math
+
+from fvstuff_amd.capi import ensemble
+
+
+def test_ensemble_ranges_and_determinism():
+    cfgs, ics, cfls = ensemble(n=256, seed=2024)
+    assert len(cfgs) == len(ics) == len(cfls) == 256
+    Ks = [c.K for c in cfgs]
+    ps = [c.p for c in cfgs]
+    assert min(Ks) >= 16 and max(Ks) <= 128 and len(set(Ks)) > 50
+    assert set(ps) == {3, 5, 7}
+    for c, ic, cfl in zip(cfgs, ics, cfls):
+        assert c.volume_flux == 1 and c.surface_flux == 1 and c.bc == 1 and c.precision == 1
+        assert 1.0 <= ic.vortex_eps <= 5.0
+        mach = math.hypot(ic.u0, ic.v0) / math.sqrt(1.4)
+        assert 0.1 - 1e-12 <= mach <= 0.8 + 1e-12
+        assert cfl == (0.5 if c.p == 3 else (0.35 if c.p == 5 else 0.25))
+    cfgs2, ics2, _ = ensemble(n=256, seed=2024)
+    assert [(c.K, c.p) for c in cfgs] == [(c.K, c.p) for c in cfgs2]
+    assert [i.vortex_eps for i in ics] == [i.vortex_eps for i in ics2]
+    cfgs3, _, _ = ensemble(n=256, seed=7)
+    assert [(c.K, c.p) for c in cfgs3] != [(c.K, c.p) for c in cfgs]
