@@ -20,6 +20,10 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # a*b+c is fused only where it is written as one expression, so every
+    # inlined copy of a helper rounds alike (a node converted in two places
+    # gives the same bits, which keeps results independent of the tiling)
+    "-ffp-contract=on",
     "-Wno-unused-command-line-argument",
 ]
 

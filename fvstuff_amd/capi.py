@@ -43,7 +43,7 @@ class DGConfig(C.Structure):
         ("av", C.c_int), ("limiter", C.c_int), ("precision", C.c_int),
         ("av_cav", C.c_double), ("av_ksensor", C.c_double),
         ("eps_rho", C.c_double), ("eps_p", C.c_double),
-        ("elems_per_block", C.c_int), ("xcd_remap", C.c_int), ("u0_prefetch", C.c_int),
+        ("elems_per_block", C.c_int), ("xcd_remap", C.c_int),
     ]
 
 
@@ -61,7 +61,7 @@ def make_config(K=50, p=3, x0=0.0, y0=0.0, L=1.0, gamma=1.4,
                 volume_flux="chandrashekar", surface_flux="hllc", bc="wall",
                 av=True, limiter=True, precision="fp64", av_cav=0.25,
                 av_ksensor=0.1, eps_rho=1e-6, eps_p=1e-6,
-                elems_per_block=0, xcd_remap=-1, u0_prefetch=-1):
+                elems_per_block=0, xcd_remap=-1):
     """A dg_config; defaults are the reference's compile-time defaults
     (dgsem_lobatto_2d.cu:61-126, Makefile DGSEM_ORDER=3, main :915-922)."""
     def enum(field, v):
@@ -70,7 +70,7 @@ def make_config(K=50, p=3, x0=0.0, y0=0.0, L=1.0, gamma=1.4,
                     enum("volume_flux", volume_flux), enum("surface_flux", surface_flux),
                     enum("bc", bc), int(bool(av)), int(bool(limiter)),
                     enum("precision", precision), float(av_cav), float(av_ksensor),
-                    float(eps_rho), float(eps_p), int(elems_per_block), int(xcd_remap), int(u0_prefetch))
+                    float(eps_rho), float(eps_p), int(elems_per_block), int(xcd_remap))
 
 
 def make_ic(kind="vortex", rho_hi=11.0, p_hi=10.0, vortex_eps=5.0, xc=5.0, yc=5.0,

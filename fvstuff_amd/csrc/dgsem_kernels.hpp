@@ -38,8 +38,14 @@ enum { ST_OK = 0, ST_NONFINITE = 3, ST_DT_UNDERFLOW = 4, ST_BLOWUP = 5 };
 constexpr int BLOCK = 256;
 
 // Device-resident step state for the fused stepping (no host round trip).
+// The stage-3 wave-speed maximum is gathered in LAM_PARTS partial slots
+// (workgroup w updates part w mod LAM_PARTS), so no single address takes one
+// atomic per workgroup; the next stage 1 folds the parts.
+constexpr int LAM_PARTS = 64;
+
 struct DevState {
-    unsigned long long lam_bits[2];  // max wave speed as ordered bit pattern (>= 0)
+    unsigned long long lam_bits[2];  // max wave speed as ordered bit pattern (>= 0), folded
+    unsigned long long lam_part[2][LAM_PARTS];   // per-slot partial maxima
     double t;                        // simulation time
     double dt;                       // dt of the current step
     int status;                      // first numerical failure (ST_*)
@@ -61,8 +67,6 @@ struct KArgs {
     int check_blowup;    // stage 1: the wave speed in slot_cur came from a step of this call
     double h, gamma, ca, cb, cfl, t_end, av_cav, av_k, eps_rho, eps_p;
     int off_U, off_av, off_lam, off_el;   // byte offsets of the LDS regions
-    int off_U0;          // LDS copy of U^n (stages 2, 3) when u0_lds
-    int u0_lds;          // prefetch U^n into LDS during the load phase
 };
 
 // Per-launch step control of the fused stepping (ping-pong wave-speed slots).
@@ -198,37 +202,10 @@ __device__ __forceinline__ void rot(Prim<Real>& w, bool swap) {
     w.v = swap ? u : v;
 }
 
-// ── volume contribution of one line: acc[i] = sum_k D_ik F#(U_i,U_k) ──────
+// ── symmetric pair loop of one line: F#(U_i,U_k) once per i < k ──────────
 template <int N, typename Real, int VOL>
-__device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Real> (&hw)[N], const Real* D,
-                                            Real (&acc)[N][4]) {
-    using Mx = M<Real>;
-    if (VOL == VOL_COLLOCATION) {
-        // linear contraction sum_k D_ik f(U_k) (:431-439)
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-            Real f[4];
-            ph.template flux<0>(doubled(hw[k]), f);
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-                const Real dik = D[i * N + k];
-#pragma unroll
-                for (int q = 0; q < 4; q++) acc[i][q] += dik * f[q];
-            }
-        }
-        return;
-    }
-    // diagonal: F#(U_i,U_i) = f(U_i) (consistency).  On LGL nodes D_ii = 0
-    // except D_00 = -p(p+1)/4 and D_pp = p(p+1)/4 (the negative-sum
-    // construction leaves ~1e-16 on the interior diagonal, dropped here).
-#pragma unroll
-    for (int i = 0; i < N; i += N - 1) {
-        Real f[4];
-        ph.template flux<0>(doubled(hw[i]), f);
-        const Real dii = D[i * N + i];
-#pragma unroll
-        for (int q = 0; q < 4; q++) acc[i][q] += dii * f[q];
-    }
+__device__ __forceinline__ void line_pairs(const Phys<Real>& ph, const Prim<Real> (&hw)[N], const Real* D,
+                                           Real (&acc)[N][4]) {
     if (VOL == VOL_RANOCHA) {
         const Real c[4] = {ph.inv_gm1, ph.inv_gm1 * (Real)(1.0 / 3.0), ph.inv_gm1 * (Real)(1.0 / 5.0),
                            ph.inv_gm1 * (Real)(1.0 / 7.0)};
@@ -264,15 +241,50 @@ __device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Rea
     }
 }
 
+// ── volume contribution of one line: acc[i] = sum_k D_ik F#(U_i,U_k) ──────
+template <int N, typename Real, int VOL>
+__device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Real> (&hw)[N], const Real* D,
+                                            Real (&acc)[N][4]) {
+    using Mx = M<Real>;
+    if (VOL == VOL_COLLOCATION) {
+        // linear contraction sum_k D_ik f(U_k) (:431-439)
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            Real f[4];
+            ph.template flux<0>(doubled(hw[k]), f);
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const Real dik = D[i * N + k];
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[i][q] += dik * f[q];
+            }
+        }
+        return;
+    }
+    // diagonal: F#(U_i,U_i) = f(U_i) (consistency).  On LGL nodes D_ii = 0
+    // except D_00 = -p(p+1)/4 and D_pp = p(p+1)/4 (the negative-sum
+    // construction leaves ~1e-16 on the interior diagonal, dropped here).
+#pragma unroll
+    for (int i = 0; i < N; i += N - 1) {
+        Real f[4];
+        ph.template flux<0>(doubled(hw[i]), f);
+        const Real dii = D[i * N + i];
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[i][q] += dii * f[q];
+    }
+    line_pairs<N, Real, VOL>(ph, hw, D, acc);
+}
+
 // Register budget: a line keeps its p+1 node states and 4(p+1) accumulators
 // live ((p+1)*9 values per lane, twice the registers in FP64).  The floor on
 // waves per SIMD lets FP64 at p >= 5 use the full 512-register file (VGPRs +
 // AGPRs) instead of spilling to scratch.
 template <int P, typename Real, int VOL, bool FULL> struct Occ { static constexpr int waves = 3; };
 template <int P, int VOL, bool FULL> struct Occ<P, double, VOL, FULL> {
-    // at p >= 5 the FP64 line state and accumulators plus one pair's
-    // temporaries exceed 256 registers: use the whole file rather than spill
-    static constexpr int waves = P >= 5 ? 1 : 2;
+    // FP64 at p >= 6 (p = 5 with the Chandrashekar flux): the line state and
+    // accumulators plus one pair's temporaries exceed 256 registers, so use
+    // the whole file rather than spill
+    static constexpr int waves = (P >= 6 || (P == 5 && VOL == VOL_CHANDRASHEKAR)) ? 1 : 2;
 };
 
 // One tile (E consecutive elements) of one stage.  `lead` marks the one tile
@@ -295,17 +307,105 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     const Real* D = (const Real*)a.tab;
     const Real* winv = D + N * N + N;
 
+    const Phys<Real> ph{(Real)g, (Real)(g - 1.0), (Real)(1.0 / (g - 1.0))};
+
+    // LDS regions
+    Real* sW = (Real*)smem;                          // [E][N2][5] prims; reused as sR [E][4][N2]
+    Real* sR = sW;
+    double* sU = (double*)(smem + a.off_U);         // [E][4][N2] FP64 stage input, then stage values
+    Real* sUc = (Real*)(smem + a.off_av);           // [E][4][N2] AV: sanitized cons
+    Real* sDu = sUc + (size_t)E * 4 * N2;           // [E][N2]
+    Real* sDv = sDu + (size_t)E * N2;               // [E][N2]
+    Real* sNu = sDv + (size_t)E * N2;               // [E]
+    double* sLam = (double*)(smem + a.off_lam);     // [E][N2]
+    double* sEl = (double*)(smem + a.off_el);       // per-element scratch
+    double* sBar = sEl;                              // [E][4]
+    double* sTrho = sEl + 4 * E;                     // [E]
+    unsigned long long* sTheta = (unsigned long long*)(sEl + 5 * E);  // [E]
+    double* sred = sEl + 6 * E;                      // [BLOCK/64]
+
+    const bool need_lam = FULL && (a.av || (a.mode == MODE_RHS && a.lam_el != nullptr));
+
+    // line owned by this thread in phase B: element le, x-line (u, v as
+    // stored) or y-line (u <-> v swapped); node k of the line is lbase + k*lstride
+    const int nlines = E * 2 * N;
+    const bool is_line = tid < nlines;
+    const int le = is_line ? tid / (2 * N) : 0;
+    const int ll = is_line ? tid - le * 2 * N : 0;
+    const bool ydir = ll >= N;
+    const int lidx = ydir ? ll - N : ll;
+    const int lbase = ydir ? lidx : lidx * N;
+    const int lstride = ydir ? N : 1;
+    const bool line_live = is_line && le < nE;
+    const Real jac = (Real)(2.0 / a.h);
+
+    // Face neighbours of the line's two end nodes: inside the tile they are
+    // read from LDS in phase B; outside it their FP64 state is loaded here,
+    // with the tile's own nodes, so that no global load waits inside the
+    // line phase.  hk: 0 = in tile, 1 = load from hn[], 2 = wall ghost.
+    int hk[2] = {0, 0}, hnd[2] = {0, 0}, hel[2] = {0, 0};
+    double Hn[2][4];
+    if (line_live) {
+        const int el = el0 + le;
+        const int ej = el / a.K, ei = el - ej * a.K;
+        const int cpos = ydir ? ej : ei;
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+            const bool edge = f ? (cpos == a.K - 1) : (cpos == 0);
+            int c2 = f ? cpos + 1 : cpos - 1;
+            if (edge) c2 = f ? 0 : a.K - 1;
+            hel[f] = ydir ? c2 * a.K + ei : ej * a.K + c2;
+            hnd[f] = lbase + (f ? 0 : N - 1) * lstride;
+            hk[f] = (edge && a.bc == BC_WALL) ? 2 : (hel[f] >= el0 && hel[f] < el0 + nE) ? 0 : 1;
+            if (hk[f] == 1) {
+                const size_t o = (size_t)hel[f] * N2 + hnd[f];
+#pragma unroll
+                for (int q = 0; q < 4; q++) Hn[f][q] = a.Up[q * S + o];
+            }
+        }
+    }
+
+    // ── A: load nodes, prims in FP64, store compute-precision prims ──
+    // A thread owns at most ceil((p+1)/2) node slots (E(p+1)^2 nodes over at
+    // least 2E(p+1) threads), s = tid + j*blockDim for j < JMAX, here and in
+    // the update phase.  All of its global loads are issued before any
+    // conversion, so each wave keeps many requests in flight.
+    constexpr int JMAX = (N + 1) / 2;
+    const bool staged = a.mode == MODE_STAGE;
+    double Ul[JMAX][4];
+#pragma unroll
+    for (int j = 0; j < JMAX; j++) {
+        const int s = tid + j * nthr;
+        if (s < nE * N2) {
+            const size_t o = (size_t)el0 * N2 + s;
+#pragma unroll
+            for (int q = 0; q < 4; q++) Ul[j][q] = a.Up[q * S + o];
+        }
+    }
     // ── fused dt (device side, :1026-1035) ──
+    // Decided after the tile's loads are issued (they are in bounds either
+    // way), so the step-state read and the loads overlap.
     double dt = 0.0;
     if (a.mode == MODE_STAGE) {
         __syncthreads();   // a persistent loop's previous tile is done with s_go / s_dt / LDS
+        unsigned long long lam_b = 0ull;
+        if (a.stage == 1 && tid < 64) {
+            // max over the partial slots (non-negative doubles and +NaN order
+            // as their unsigned bit patterns)
+            lam_b = a.st->lam_part[sc.slot_cur][tid & (LAM_PARTS - 1)];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const unsigned long long v = __shfl_xor(lam_b, o, 64);
+                lam_b = v > lam_b ? v : lam_b;
+            }
+        }
         if (tid == 0) {
             DevState* st = a.st;
             int go = 0;
             double d = 0.0;
             if (st->status == ST_OK) {
                 if (a.stage == 1) {
-                    const double lam = bitsd(st->lam_bits[sc.slot_cur]);
+                    const double lam = bitsd(lam_b);
                     const double t = st->t;
                     int status = ST_OK, skip = 0;
                     if (a.t_end > 0.0 && t >= a.t_end - 1e-12 * fmax(1.0, fabs(a.t_end))) {
@@ -333,7 +433,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                     if (a.stage == 2 && lead) {
                         st->t += d;
                         st->steps += 1;
-                        st->lam_bits[sc.slot_next] = 0ull;
+                        for (int k = 0; k < LAM_PARTS; k++) st->lam_part[sc.slot_next][k] = 0ull;
                     }
                 }
             }
@@ -345,73 +445,25 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         dt = s_dt;
     }
 
-    const Phys<Real> ph{(Real)g, (Real)(g - 1.0), (Real)(1.0 / (g - 1.0))};
-
-    // LDS regions
-    Real* sW = (Real*)smem;                          // [E][N2][5] prims; reused as sR [E][4][N2]
-    Real* sR = sW;
-    double* sU = (double*)(smem + a.off_U);         // [E][4][N2] FP64 stage input, then stage values
-    Real* sUc = (Real*)(smem + a.off_av);           // [E][4][N2] AV: sanitized cons
-    Real* sDu = sUc + (size_t)E * 4 * N2;           // [E][N2]
-    Real* sDv = sDu + (size_t)E * N2;               // [E][N2]
-    Real* sNu = sDv + (size_t)E * N2;               // [E]
-    double* sLam = (double*)(smem + a.off_lam);     // [E][N2]
-    double* sEl = (double*)(smem + a.off_el);       // per-element scratch
-    double* sBar = sEl;                              // [E][4]
-    double* sTrho = sEl + 4 * E;                     // [E]
-    unsigned long long* sTheta = (unsigned long long*)(sEl + 5 * E);  // [E]
-    double* sred = sEl + 6 * E;                      // [BLOCK/64]
-
-    const bool need_lam = FULL && (a.av || (a.mode == MODE_RHS && a.lam_el != nullptr));
-
-    // ── A: load nodes, prims in FP64, store compute-precision prims ──
-    // A thread owns at most ceil((p+1)/2) node slots (E(p+1)^2 nodes over at
-    // least 2E(p+1) threads).  All of its global loads - the stage input and,
-    // for stages 2 and 3, U^n needed only by the update - are issued before
-    // any conversion, so each wave keeps many requests in flight.
-    constexpr int JMAX = (N + 1) / 2;
-    const bool staged = a.mode == MODE_STAGE;
-    // (FP32 form only: the FP64 form has no registers to spare for it)
-    const bool pre0 = sizeof(Real) == 4 && staged && a.u0_lds && a.ca != 0.0;
-    double* sU0 = (double*)(smem + a.off_U0);       // [E][4][N2] U^n copy
-    double Ul[JMAX][4], U0l[JMAX][4];
-#pragma unroll
-    for (int j = 0; j < JMAX; j++) {
-        const int s = tid + j * nthr;
-        const int e = s / N2, nd = s - e * N2;
-        if (s < E * N2 && e < nE) {
-            const size_t o = (size_t)(el0 + e) * N2 + nd;
-#pragma unroll
-            for (int q = 0; q < 4; q++) Ul[j][q] = a.Up[q * S + o];
-            if (pre0) {
-#pragma unroll
-                for (int q = 0; q < 4; q++) U0l[j][q] = a.U0[q * S + o];
-            }
-        }
-    }
 #pragma unroll
     for (int j = 0; j < JMAX; j++) {
         const int s = tid + j * nthr;
         if (s >= E * N2) continue;
         const int e = s / N2, nd = s - e * N2;
         double W[4] = {1.0, 0.0, 0.0, 1.0};
-        if (e < nE) {
+        if (s < nE * N2) {
             if (staged) {
 #pragma unroll
                 for (int q = 0; q < 4; q++) sU[(size_t)e * 4 * N2 + q * N2 + nd] = Ul[j][q];
             }
-            if (pre0) {
-#pragma unroll
-                for (int q = 0; q < 4; q++) sU0[(size_t)e * 4 * N2 + q * N2 + nd] = U0l[j][q];
-            }
             Cons64::to_prim(Ul[j], W, g);
         }
         Prim<Real> w = make_prim<Real>(W);
-        Real* p = sW + ((size_t)e * N2 + nd) * 5;
+        Real* p = sW + (size_t)s * 5;
         p[0] = (Real)0.5 * w.r; p[1] = (Real)0.5 * w.u; p[2] = (Real)0.5 * w.v;
         p[3] = (Real)0.5 * w.p; p[4] = (Real)0.5 * w.z;
         if (FULL) {
-            if (need_lam) sLam[e * N2 + nd] = Cons64::wave_speed(W, g);
+            if (need_lam) sLam[s] = Cons64::wave_speed(W, g);
             if (a.av) {
                 Real* c = sUc + (size_t)e * 4 * N2 + nd;
                 c[0] = w.r; c[N2] = w.r * w.u; c[2 * N2] = w.r * w.v;
@@ -420,17 +472,6 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         }
     }
     __syncthreads();
-
-    const int nlines = E * 2 * N;
-    const bool is_line = tid < nlines;
-    const int le = is_line ? tid / (2 * N) : 0;       // element of this line
-    const int ll = is_line ? tid - le * 2 * N : 0;
-    const bool ydir = ll >= N;                          // y-line (u,v swapped)
-    const int lidx = ydir ? ll - N : ll;
-    const int lbase = ydir ? lidx : lidx * N;           // node index of the line's k-th node:
-    const int lstride = ydir ? N : 1;                   //   lbase + k*lstride
-    const bool line_live = is_line && le < nE;
-    const Real jac = (Real)(2.0 / a.h);
 
     // ── AV pre-pass: velocity divergence, sensor, element viscosity (:483-521) ──
     if (FULL && a.av) {
@@ -484,40 +525,31 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             hw[k].r = p[0]; hw[k].u = p[1]; hw[k].v = p[2]; hw[k].p = p[3]; hw[k].z = p[4];
             rot(hw[k], ydir);
         }
-        line_volume<N, Real, VOL>(ph, hw, D, acc);
-        const Real scale = VOL == VOL_COLLOCATION ? -jac : (Real)-2 * jac;
-#pragma unroll
-        for (int i = 0; i < N; i++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) acc[i][q] *= scale;
 
-        // ── faces at both ends of the line (one call site for the Riemann flux) ──
+        // ── surface terms at both ends of the line (one call site for the
+        //    Riemann flux), evaluated first: their neighbour states are ready
+        Real cm[4] = {0, 0, 0, 0}, cp[4] = {0, 0, 0, 0};
         if (line_live) {
-            const int el = el0 + le;
-            const int ej = el / a.K, ei = el - ej * a.K;
-            const int cpos = ydir ? ej : ei;
-            Real cm[4], cp[4];
 #pragma unroll 1
             for (int f = 0; f < 2; f++) {
                 const Prim<Real> own = doubled(f ? hw[N - 1] : hw[0]);
-                const bool edge = f ? (cpos == a.K - 1) : (cpos == 0);
+                const int kind = f ? hk[1] : hk[0];
                 Prim<Real> nb;
-                if (edge && a.bc == BC_WALL) {
+                if (kind == 2) {
                     nb = wall_ghost(own, 0);
                 } else {
-                    int c2 = f ? cpos + 1 : cpos - 1;
-                    if (edge) c2 = f ? 0 : a.K - 1;
-                    const int elnb = ydir ? c2 * a.K + ei : ej * a.K + c2;
-                    const int ndnb = lbase + (f ? 0 : N - 1) * lstride;
-                    if (elnb >= el0 && elnb < el0 + nE) {
+                    if (kind == 0) {
                         // neighbour staged by this tile: the same FP64 -> prim
                         // conversion, so the face sees bit-identical states
+                        const int elnb = f ? hel[1] : hel[0], ndnb = f ? hnd[1] : hnd[0];
                         const Real* pn = sW + ((size_t)(elnb - el0) * N2 + ndnb) * 5;
                         nb.r = pn[0]; nb.u = pn[1]; nb.v = pn[2]; nb.p = pn[3]; nb.z = pn[4];
                         nb = doubled(nb);
                     } else {
-                        double W[4];
-                        load_prim64(a.Up, S, (size_t)elnb * N2 + ndnb, g, W);
+                        double U[4], W[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) U[q] = f ? Hn[1][q] : Hn[0][q];
+                        Cons64::to_prim(U, W, g);
                         nb = make_prim<Real>(W);
                     }
                     rot(nb, ydir);
@@ -534,6 +566,15 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                     if (f) cp[q] = v; else cm[q] = v;
                 }
             }
+        }
+
+        line_volume<N, Real, VOL>(ph, hw, D, acc);
+        const Real scale = VOL == VOL_COLLOCATION ? -jac : (Real)-2 * jac;
+#pragma unroll
+        for (int i = 0; i < N; i++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[i][q] *= scale;
+        if (line_live) {
 #pragma unroll
             for (int q = 0; q < 4; q++) { acc[0][q] += cm[q]; acc[N - 1][q] += cp[q]; }
         }
@@ -579,6 +620,20 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             acc[i][2] = ydir ? m1 : m2;
         }
     }
+    // U^n for the update (stages 2, 3), issued now so that the loads are in
+    // flight across the barriers and the LDS exchange below
+    double U0l[JMAX][4];
+    if (staged && a.ca != 0.0) {
+#pragma unroll
+        for (int j = 0; j < JMAX; j++) {
+            const int s = tid + j * nthr;
+            if (s < nE * N2) {
+                const size_t o = (size_t)el0 * N2 + s;
+#pragma unroll
+                for (int q = 0; q < 4; q++) U0l[j][q] = a.U0[q * S + o];
+            }
+        }
+    }
     __syncthreads();   // every read of sW is done: reuse it as sR
     if (is_line && !ydir) {
         Real* r = sR + (size_t)le * 4 * N2;
@@ -618,20 +673,18 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     const bool lim = FULL && a.limiter;
     const bool want_lambda = a.stage == 3;
     double lmax = 0.0;
-    for (int s = tid; s < nE * N2; s += nthr) {
+#pragma unroll
+    for (int j = 0; j < JMAX; j++) {
+        const int s = tid + j * nthr;
+        if (s >= nE * N2) continue;
         const int e = s / N2, nd = s - e * N2;
-        const size_t o = (size_t)(el0 + e) * N2 + nd;
+        const size_t o = (size_t)el0 * N2 + s;
         const Real* r = sR + (size_t)e * 4 * N2 + nd;
         double* up = sU + (size_t)e * 4 * N2 + nd;
         double Us[4];
-        if (pre0) {
-            const double* u0 = sU0 + (size_t)e * 4 * N2 + nd;
+        if (a.ca != 0.0) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) Us[q] = a.ca * u0[q * N2] + a.cb * (up[q * N2] + dt * (double)r[q * N2]);
-        } else if (a.ca != 0.0) {
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                Us[q] = a.ca * a.U0[q * S + o] + a.cb * (up[q * N2] + dt * (double)r[q * N2]);
+            for (int q = 0; q < 4; q++) Us[q] = a.ca * U0l[j][q] + a.cb * (up[q * N2] + dt * (double)r[q * N2]);
         } else {
 #pragma unroll
             for (int q = 0; q < 4; q++) Us[q] = up[q * N2] + dt * (double)r[q * N2];
@@ -639,14 +692,14 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         Cons64::sanitize_cons(Us, g);
         if (lim) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) sU[(size_t)e * 4 * N2 + q * N2 + nd] = Us[q];
+            for (int q = 0; q < 4; q++) up[q * N2] = Us[q];
         } else {
 #pragma unroll
             for (int q = 0; q < 4; q++) a.out[q * S + o] = Us[q];
             if (want_lambda) {
-                double W[4];
-                Cons64::to_prim(Us, W, g);
-                lmax = nanmax(lmax, Cons64::wave_speed(W, g));
+                double W[4], ri;
+                Cons64::to_prim(Us, W, g, ri);
+                lmax = nanmax(lmax, Cons64::wave_speed(W, g, ri));
             }
         }
     }
@@ -654,18 +707,18 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         limit_block<N>(sU, a.w64, nE, g, a.eps_rho, a.eps_p, sBar, sTrho, sTheta);
         for (int s = tid; s < nE * N2; s += nthr) {
             const int e = s / N2, nd = s - e * N2;
-            const size_t o = (size_t)(el0 + e) * N2 + nd;
+            const size_t o = (size_t)el0 * N2 + s;
             double Us[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) { Us[q] = sU[(size_t)e * 4 * N2 + q * N2 + nd]; a.out[q * S + o] = Us[q]; }
             if (want_lambda) {
-                double W[4];
-                Cons64::to_prim(Us, W, g);
-                lmax = nanmax(lmax, Cons64::wave_speed(W, g));
+                double W[4], ri;
+                Cons64::to_prim(Us, W, g, ri);
+                lmax = nanmax(lmax, Cons64::wave_speed(W, g, ri));
             }
         }
     }
-    if (want_lambda) block_max_atomic(lmax, &a.st->lam_bits[sc.slot_next], sred);
+    if (want_lambda) block_max_atomic(lmax, &a.st->lam_part[sc.slot_next][blockIdx.x & (LAM_PARTS - 1)], sred);
 }
 
 template <int P, typename Real, bool FULL, int VOL>
@@ -703,14 +756,13 @@ __global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_ke
 
 // Host-side LDS layout shared by the launcher and the kernel.
 struct LdsLayout {
-    int off_U, off_av, off_lam, off_el, off_U0, total;
+    int off_U, off_av, off_lam, off_el, total;
 };
 
 inline int align_up(int x, int a) { return (x + a - 1) / a * a; }
 
-// `stage`: the FP64 input state is staged in LDS (sU) for the update phase;
-// `u0`: U^n is prefetched into LDS too.
-inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, bool lam, bool u0 = false) {
+// `stage`: the FP64 input state is staged in LDS (sU) for the update phase.
+inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, bool lam) {
     const int N2 = N * N;
     LdsLayout L;
     int off = align_up(E * N2 * 5 * real_bytes, 16);
@@ -722,8 +774,6 @@ inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, b
     if (lam || av) off += align_up(E * N2 * 8, 16);
     L.off_el = off;
     off += align_up((6 * E + BLOCK / 64) * 8, 16);
-    L.off_U0 = off;
-    if (u0) off += align_up(E * 4 * N2 * 8, 16);
     L.total = off;
     return L;
 }

@@ -29,7 +29,6 @@ struct dg_ctx {
     double* d_xi = nullptr;              // LGL node coordinates (IC kernel)
     int part_blocks = 0;
     int E = 0, xcd = 0;
-    int u0_lds = 0;                      // prefetch U^n into LDS (tuned)
     int last_steps = 0;                  // steps taken by the last dg_advance
     // graph replay of the fused loop: two steps per replay, captured on `cap`
     hipStream_t cap = nullptr;
@@ -206,8 +205,9 @@ __global__ void rk3_kernel(double* Uo, const double* U0, const double* Up, const
     }
 }
 
-// Max wave speed of a state, for the first step of the fused stepping.
-__global__ void lambda_kernel(const double* Q, size_t S, double g, unsigned long long* dst) {
+// Max wave speed of a state, for the first step of the fused stepping
+// (into the partial slots `parts`, as the stage-3 kernels do).
+__global__ void lambda_kernel(const double* Q, size_t S, double g, unsigned long long* parts) {
     __shared__ double sred[BLOCK / 64];
     double m = 0.0;
     for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < S; n += (size_t)blockDim.x * gridDim.x) {
@@ -215,8 +215,19 @@ __global__ void lambda_kernel(const double* Q, size_t S, double g, unsigned long
         load_prim64(Q, S, n, g, W);
         m = nanmax(m, Cons64::wave_speed(W, g));
     }
-    block_max_atomic(m, dst, sred);
+    block_max_atomic(m, &parts[blockIdx.x & (LAM_PARTS - 1)], sred);
 }
+
+// lam_bits[s] = max of slot s's partial maxima (host read-back after stepping)
+__device__ __forceinline__ void fold_lambda(DevState* st) {
+    for (int s = 0; s < 2; s++) {
+        unsigned long long m = 0ull;
+        for (int k = 0; k < LAM_PARTS; k++) m = st->lam_part[s][k] > m ? st->lam_part[s][k] : m;
+        st->lam_bits[s] = m;
+    }
+}
+
+__global__ void fold_lambda_kernel(DevState* st) { fold_lambda(st); }
 
 // max of an array (reduce_max/gpu_max, :646-659) into one bit-pattern slot
 __global__ void reduce_max_kernel(const double* v, int n, unsigned long long* dst) {
@@ -226,9 +237,14 @@ __global__ void reduce_max_kernel(const double* v, int n, unsigned long long* ds
     block_max_atomic(m, dst, sred);
 }
 
-__global__ void state_reset_kernel(DevState* st, double t0) {
+__device__ __forceinline__ void clear_lambda(DevState* st) {
     st->lam_bits[0] = 0ull;
     st->lam_bits[1] = 0ull;
+    for (int k = 0; k < LAM_PARTS; k++) st->lam_part[0][k] = st->lam_part[1][k] = 0ull;
+}
+
+__global__ void state_reset_kernel(DevState* st, double t0) {
+    clear_lambda(st);
     st->t = t0;
     st->dt = 0.0;
     st->status = ST_OK;
@@ -321,8 +337,7 @@ static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd, hipStream
     const bool lam = s.mode == MODE_RHS && s.lam_el != nullptr;
     const bool lim = s.mode == MODE_STAGE && f.limiter;
     const bool full = f.av || lim || lam;
-    const bool u0 = s.mode == MODE_STAGE && c->u0_lds;
-    LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, s.mode == MODE_STAGE, f.av != 0, lam, u0);
+    LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, s.mode == MODE_STAGE, f.av != 0, lam);
     KArgs a;
     memset(&a, 0, sizeof(a));
     a.Up = s.Up; a.U0 = s.U0; a.out = s.out; a.lam_el = s.lam_el;
@@ -336,7 +351,6 @@ static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd, hipStream
     a.h = c->h; a.gamma = f.gamma; a.ca = s.ca; a.cb = s.cb; a.cfl = s.cfl; a.t_end = s.t_end;
     a.av_cav = f.av_cav; a.av_k = f.av_ksensor; a.eps_rho = f.eps_rho; a.eps_p = f.eps_p;
     a.off_U = L.off_U; a.off_av = L.off_av; a.off_lam = L.off_lam; a.off_el = L.off_el;
-    a.off_U0 = L.off_U0; a.u0_lds = u0;
     const int grid = (c->NEL + E - 1) / E;
     const int block = std::min(BLOCK, (E * 2 * c->N + 63) / 64 * 64);
     HIPCK(c, stage_launcher(f.p)(a, fp32, full, grid, block, (size_t)L.total, stream));
@@ -350,8 +364,7 @@ __global__ void batch_reset_kernel(const KArgs* args, int n, const double* t0) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     DevState* st = args[3 * i].st;
-    st->lam_bits[0] = 0ull;
-    st->lam_bits[1] = 0ull;
+    clear_lambda(st);
     st->t = t0 ? t0[i] : 0.0;
     st->dt = 0.0;
     st->status = ST_OK;
@@ -375,13 +388,16 @@ __global__ void batch_lambda_kernel(const KArgs* args, const int2* tiles, int nt
             m = nanmax(m, Cons64::wave_speed(W, a.gamma));
         }
         __syncthreads();   // sred reuse across tiles
-        block_max_atomic(m, &a.st->lam_bits[0], sred);
+        block_max_atomic(m, &a.st->lam_part[0][blockIdx.x & (LAM_PARTS - 1)], sred);
     }
 }
 
 __global__ void batch_gather_kernel(const KArgs* args, int n, DevState* out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = *args[3 * i].st;
+    if (i < n) {
+        fold_lambda(args[3 * i].st);
+        out[i] = *args[3 * i].st;
+    }
 }
 
 // ── exported API ───────────────────────────────────────────────────────────
@@ -394,7 +410,7 @@ void dg_config_default(dg_config* f) {
     f->volume_flux = DG_VOL_CHANDRASHEKAR; f->surface_flux = DG_SURF_HLLC; f->bc = DG_BC_WALL;
     f->av = 1; f->limiter = 1; f->precision = DG_FP64;
     f->av_cav = 0.25; f->av_ksensor = 0.1; f->eps_rho = 1e-6; f->eps_p = 1e-6;
-    f->elems_per_block = 0; f->xcd_remap = -1; f->u0_prefetch = -1;
+    f->elems_per_block = 0; f->xcd_remap = -1;
 }
 
 void dg_ic_default(dg_ic_params* ic, int kind) {
@@ -465,7 +481,6 @@ int dg_create(const dg_config* cfg, void* stream, dg_ctx** out) {
     const int Emax = max_elems_per_block(c->N);
     c->E = cfg->elems_per_block > 0 ? std::min(cfg->elems_per_block, Emax) : default_E(c->N, c->NEL);
     c->xcd = cfg->xcd_remap >= 0 ? (cfg->xcd_remap != 0) : 1;
-    c->u0_lds = cfg->u0_prefetch > 0;
 
     // tables: D[N2], w[N], winv[N]
     std::vector<double> t64(c->N2 + 2 * c->N);
@@ -581,7 +596,7 @@ static int advance_impl(dg_ctx* c, double* Q, int nsteps, double cfl, double t_e
     const double t0 = t_inout ? *t_inout : 0.0;
     hipLaunchKernelGGL(state_reset_kernel, dim3(1), dim3(1), 0, c->stream, c->d_st, t0);
     hipLaunchKernelGGL(lambda_kernel, dim3(grid_for(S, BLOCK)), dim3(BLOCK), 0, c->stream, Q, S, c->cfg.gamma,
-                       &c->d_st->lam_bits[0]);
+                       c->d_st->lam_part[0]);
     HIPCK(c, hipGetLastError());
     int rc = DG_OK, n = 0;
     if (nsteps > 0 && (rc = advance_step(c, Q, n++, cfl, t_end, E, xcd, c->stream))) return rc;
@@ -611,6 +626,8 @@ static int advance_impl(dg_ctx* c, double* Q, int nsteps, double cfl, double t_e
     }
     for (; n < nsteps; n++)
         if ((rc = advance_step(c, Q, n, cfl, t_end, E, xcd, c->stream))) return rc;
+    hipLaunchKernelGGL(fold_lambda_kernel, dim3(1), dim3(1), 0, c->stream, c->d_st);
+    HIPCK(c, hipGetLastError());
     DevState h;
     HIPCK(c, hipMemcpyAsync(&h, c->d_st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
@@ -685,13 +702,11 @@ int dg_autotune(dg_ctx* c, int* elems_per_block, int* xcd_remap) {
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     float best = 1e30f;
-    int bestE = c->E, bestX = c->xcd, bestU = c->u0_lds;
+    int bestE = c->E, bestX = c->xcd;
     // every candidate starts from the same smooth state and takes a few small
     // steps (CFL 0.1 is stable for every degree), so numerics never interfere
-    for (int cand = 0; cand < 4 && rc == DG_OK; cand++)
+    for (int x = 0; x < 2 && rc == DG_OK; x++)
         for (int E : Es) {
-            const int x = cand & 1;
-            c->u0_lds = cand >> 1;
             double t = 0.0;
             if ((rc = dg_init_state(c, &ic, Q))) break;
             rc = advance_impl(c, Q, 1, 0.1, 0.0, &t, nullptr, E, x, false);   // warm-up (LDS attribute, caches)
@@ -703,7 +718,7 @@ int dg_autotune(dg_ctx* c, int* elems_per_block, int* xcd_remap) {
             if (rc) break;
             float ms = 0.f;
             hipEventElapsedTime(&ms, e0, e1);
-            if (ms < best) { best = ms; bestE = E; bestX = x; bestU = c->u0_lds; }
+            if (ms < best) { best = ms; bestE = E; bestX = x; }
         }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -711,7 +726,6 @@ int dg_autotune(dg_ctx* c, int* elems_per_block, int* xcd_remap) {
     if (rc) return rc;
     c->E = bestE;
     c->xcd = bestX;
-    c->u0_lds = bestU;
     if (elems_per_block) *elems_per_block = bestE;
     if (xcd_remap) *xcd_remap = bestX;
     return DG_OK;
@@ -796,7 +810,6 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
             a.cfl = cfl[i]; a.t_end = t_end;
             a.av_cav = f.av_cav; a.av_k = f.av_ksensor; a.eps_rho = f.eps_rho; a.eps_p = f.eps_p;
             a.off_U = L.off_U; a.off_av = L.off_av; a.off_lam = L.off_lam; a.off_el = L.off_el;
-            a.off_U0 = L.off_U0; a.u0_lds = 0;
             b->h_args[(size_t)i * 3 + (s - 1)] = a;
         }
     }

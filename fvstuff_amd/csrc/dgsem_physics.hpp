@@ -73,8 +73,13 @@ struct Cons64 {
     // cons_to_prim_sane: sanitize, convert, floor rho/p, clamp |u|,|v| <= 1e4
     // (one reciprocal of rho serves u, v and the kinetic energy)
     static __device__ __forceinline__ void to_prim(const double U[4], double W[4], double g) {
+        double ri;
+        to_prim(U, W, g, ri);
+    }
+    // the same, also returning ri = 1/W[0]
+    static __device__ __forceinline__ void to_prim(const double U[4], double W[4], double g, double& ri) {
         const double r = fmax(U[0], 1e-14);
-        const double ri = rcp(r);
+        ri = rcp(r);
         const double ke = 0.5 * (U[1] * U[1] + U[2] * U[2]) * ri;
         double E = U[3];
         if ((g - 1.0) * (E - ke) < 1e-14) E = 1e-14 / (g - 1.0) + ke;
@@ -89,6 +94,12 @@ struct Cons64 {
     // max wave speed |u|+|v|+c of sanitized primitives (:483-486)
     static __device__ __forceinline__ double wave_speed(const double W[4], double g) {
         double c2 = g * W[3] * rcp(fmax(W[0], 1e-12));
+        return fabs(W[1]) + fabs(W[2]) + sqrt(fmax(c2, 1e-14));
+    }
+    // the same with ri = 1/W[0] from to_prim (bit-identical: the reciprocal
+    // of max(rho, 1e-12) is ri whenever rho >= 1e-12)
+    static __device__ __forceinline__ double wave_speed(const double W[4], double g, double ri) {
+        double c2 = g * W[3] * (W[0] >= 1e-12 ? ri : rcp(1e-12));
         return fabs(W[1]) + fabs(W[2]) + sqrt(fmax(c2, 1e-14));
     }
 };
@@ -163,7 +174,9 @@ template <typename Real> struct Phys {
     // form the line loop uses: (a+b)/2 = aL/2 + aR/2 needs no multiply, one
     // reciprocal of the product (rho_L+rho_R)(z_L+z_R) serves both log means,
     // and c[] = 1/(g-1) * (1, 1/3, 1/5, 1/7) folds the energy factor into the
-    // series of the inverse log mean of z = rho/p.
+    // series of the inverse log mean of z = rho/p.  The ratio-form log replaces
+    // the series per lane where the series does not apply; the result does not
+    // depend on how lines share waves.
     __device__ __forceinline__ static void ranocha_half(const Prim<Real>& L, const Prim<Real>& R, const Real c[4],
                                                         Real inv_gm1, Real F[4]) {
         const Real sr = L.r + R.r, sz = L.z + R.z;          // (rho_L+rho_R)/2, (z_L+z_R)/2
@@ -174,11 +187,9 @@ template <typename Real> struct Phys {
         const Real fz = dz * iz, uz = fz * fz;
         Real r_ln = sr * ((Real)1 - ur * ((Real)(1.0 / 3.0) + ur * ((Real)(4.0 / 45.0) + ur * (Real)(44.0 / 945.0))));
         Real izg = iz * (c[0] + uz * (c[1] + uz * (c[2] + uz * c[3])));    // 1/((g-1) z_ln)
-        // ratio-form log where the series does not apply.  The branch is
-        // wave-uniform (a divergent one would make the register allocator keep
-        // both sides' values of every unrolled pair live) and each lane then
-        // selects its own value, so results do not depend on how lines are
-        // grouped into waves.
+        // wave-uniform branch (a divergent one makes the register allocator
+        // keep both sides' values of every unrolled pair live); each lane then
+        // selects its own value
         const bool lr = ur >= (Real)Mx::series_thr, lz = uz >= (Real)Mx::series_thr;
         if (__any(lr || lz)) {
             const Real r_ex = Mx::div((Real)2 * dr, Mx::ln(Mx::div(L.r, R.r)));   // dr is (rho_L-rho_R)/2

@@ -76,8 +76,6 @@ typedef struct dg_config {
     double eps_p;
     int    elems_per_block; /* tuning: elements per workgroup, 0 = heuristic     */
     int    xcd_remap;       /* tuning: XCD-aware block->tile map, -1 = default on */
-    int    u0_prefetch;     /* tuning: stage U^n in LDS during the load phase
-                               (FP32 form), -1 = default off                      */
 } dg_config;
 
 /* Initial-condition parameters (unused fields are ignored). */
@@ -173,8 +171,8 @@ int         dg_sensor(dg_ctx* ctx, const double* Q, double* sensor_el);
  * entropy -rho*s/(gamma-1), s = ln(p*rho^-gamma). */
 int         dg_totals(dg_ctx* ctx, const double* Q, double out[5]);
 
-/* Tuning: time a few candidate schedules (elements per block, XCD remap,
- * U^n prefetch) on scratch buffers and keep the fastest for this ctx.  Optional; dg_create
+/* Tuning: time a few candidate schedules (elements per block x XCD remap)
+ * on scratch buffers and keep the fastest for this ctx.  Optional; dg_create
  * picks a heuristic schedule.  Writes the chosen values if pointers non-NULL. */
 int         dg_autotune(dg_ctx* ctx, int* elems_per_block, int* xcd_remap);
 
