@@ -27,6 +27,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per stage launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of the
+# stage kernel (scaled by a device copy's readings), keyed by (K, p, form);
+# the runs and the scaling are listed in the named profile.
+PMC_TRAFFIC = {(512, 7, "mixed"): (1.457e9, "profiles/stage_kernel_p7_vortex512.md")}
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -301,6 +305,7 @@ def main():
         cpu = cpu_baseline(cfgc, ic, args.cpu_steps, args.cfl)
         log(f"[bench] cpu baseline: {cpu}")
 
+    traffic = PMC_TRAFFIC.get((K, p, main_form))
     out = {
         "metric": "DOF*stage updates/s, mixed vs FP64 DGSEM at equal L2 error; % HBM peak",
         "value": res[main_form]["rate"],
@@ -321,7 +326,8 @@ def main():
         "mixed_over_fp64": (res["mixed"]["rate"] / res["fp64"]["rate"]) if ("mixed" in res and "fp64" in res) else None,
         "accuracy": acc,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic[0] if traffic else None,
+                     "traffic_source": traffic[1] if traffic else None,
                      "kernel": f"stage_kernel<{p},{'float' if main_form == 'mixed' else 'double'},lean,Ranocha>",
                      "bytes_per_dof_stage": b_alg, "avg_launch_ms": ms_launch},
         "cpu_baseline": cpu,

@@ -202,6 +202,24 @@ __device__ __forceinline__ void rot(Prim<Real>& w, bool swap) {
     w.v = swap ? u : v;
 }
 
+// FP64: materialise a pair's two accumulator rows where the pair is
+// evaluated (an empty asm that reads and writes them, no instruction).
+// Without it the compiler sinks every pair's flux and accumulation past the
+// later pairs' log-branches into one block at the end of the line, keeping
+// all pairs' intermediate values live at once: at p = 7 that needs the whole
+// register file (1 wave per SIMD) instead of 218 registers (2 waves).  The
+// FP32 form keeps the sunk schedule, whose wide independent block runs
+// faster at its occupancy (measured).
+template <int N, typename Real>
+__device__ __forceinline__ void pin_pair(Real (&acc)[N][4], int i, int k) {
+    if (sizeof(Real) != 8) return;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        asm volatile("" : "+v"(acc[i][q]));
+        asm volatile("" : "+v"(acc[k][q]));
+    }
+}
+
 // ── symmetric pair loop of one line: F#(U_i,U_k) once per i < k ──────────
 template <int N, typename Real, int VOL>
 __device__ __forceinline__ void line_pairs(const Phys<Real>& ph, const Prim<Real> (&hw)[N], const Real* D,
@@ -221,6 +239,7 @@ __device__ __forceinline__ void line_pairs(const Phys<Real>& ph, const Prim<Real
                     acc[i][q] += dik * F[q];
                     acc[k][q] += dki * F[q];
                 }
+                pin_pair<N>(acc, i, k);
             }
         }
         return;
@@ -237,6 +256,7 @@ __device__ __forceinline__ void line_pairs(const Phys<Real>& ph, const Prim<Real
                 acc[i][q] += dik * F[q];
                 acc[k][q] += dki * F[q];
             }
+            pin_pair<N>(acc, i, k);
         }
     }
 }

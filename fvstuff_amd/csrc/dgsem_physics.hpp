@@ -39,8 +39,9 @@ template <> struct M<float> {
     static __device__ __forceinline__ float fmax_(float a, float b) { return fmaxf(a, b); }
     static __device__ __forceinline__ float fmin_(float a, float b) { return fminf(a, b); }
     static __device__ __forceinline__ float abs_(float a) { return fabsf(a); }
-    // the 4-term series is accurate to 3e-10 at u = 1e-2, far below FP32 epsilon
-    static constexpr float series_thr = 1e-2f;
+    // below 5e-3 the series truncated after its u^2 term is accurate to 2e-8
+    // (the dropped terms are 44u^3/945 and u^3/7), under half an FP32 ulp
+    static constexpr float series_thr = 5e-3f;
 };
 
 // Primitive node state held in registers / LDS: rho, u, v, p and z = rho/p.
@@ -139,8 +140,9 @@ template <typename Real> struct Phys {
     // Log mean L(a,b) = (a-b)/ln(a/b) and its inverse.  With f = (a-b)/(a+b),
     // ln(a/b) = 2 atanh(f), so L = (a+b)/2 * f/atanh(f); for small f the
     // series atanh(f)/f = 1 + f^2/3 + f^4/5 + f^6/7 (the reference's form,
-    // :244-253) and its reciprocal 1 - f^2/3 - 4f^4/45 - 44f^6/945 are used.
-    // The log branch keeps the ratio form ln(a/b), well conditioned in FP32.
+    // :244-253) and its reciprocal 1 - f^2/3 - 4f^4/45 - 44f^6/945 are used
+    // (in FP32 without their f^6 terms, below a smaller threshold).  The log
+    // branch keeps the ratio form ln(a/b), well conditioned in FP32.
     // Log means of one pair: the series form first (no log, one reciprocal
     // each), replaced by the ratio-form log only where the series does not
     // apply.  That branch is skipped by the whole wave on smooth data.
@@ -150,10 +152,15 @@ template <typename Real> struct Phys {
         Real ra = Mx::rcp(sa), rb = Mx::rcp(sb);
         Real fa = (aL - aR) * ra, ua = fa * fa;
         Real fb = (bL - bR) * rb, ub = fb * fb;
-        a_ln = (Real)0.5 * sa *
-               ((Real)1 - ua * ((Real)(1.0 / 3.0) + ua * ((Real)(4.0 / 45.0) + ua * (Real)(44.0 / 945.0))));
-        inv_b_ln = (Real)2 * rb *
-               ((Real)1 + ub * ((Real)(1.0 / 3.0) + ub * ((Real)(1.0 / 5.0) + ub * (Real)(1.0 / 7.0))));
+        if (sizeof(Real) == 4) {
+            a_ln = (Real)0.5 * sa * ((Real)1 - ua * ((Real)(1.0 / 3.0) + ua * (Real)(4.0 / 45.0)));
+            inv_b_ln = (Real)2 * rb * ((Real)1 + ub * ((Real)(1.0 / 3.0) + ub * (Real)(1.0 / 5.0)));
+        } else {
+            a_ln = (Real)0.5 * sa *
+                   ((Real)1 - ua * ((Real)(1.0 / 3.0) + ua * ((Real)(4.0 / 45.0) + ua * (Real)(44.0 / 945.0))));
+            inv_b_ln = (Real)2 * rb *
+                   ((Real)1 + ub * ((Real)(1.0 / 3.0) + ub * ((Real)(1.0 / 5.0) + ub * (Real)(1.0 / 7.0))));
+        }
         if (ua >= (Real)Mx::series_thr) a_ln = Mx::div(aL - aR, Mx::ln(Mx::div(aL, aR)));
         if (ub >= (Real)Mx::series_thr) inv_b_ln = Mx::div(Mx::ln(Mx::div(bL, bR)), bL - bR);
     }
@@ -185,8 +192,14 @@ template <typename Real> struct Phys {
         const Real dr = L.r - R.r, dz = L.z - R.z;
         const Real fr = dr * ir, ur = fr * fr;
         const Real fz = dz * iz, uz = fz * fz;
-        Real r_ln = sr * ((Real)1 - ur * ((Real)(1.0 / 3.0) + ur * ((Real)(4.0 / 45.0) + ur * (Real)(44.0 / 945.0))));
-        Real izg = iz * (c[0] + uz * (c[1] + uz * (c[2] + uz * c[3])));    // 1/((g-1) z_ln)
+        Real r_ln, izg;                                     // rho_ln, 1/((g-1) z_ln)
+        if (sizeof(Real) == 4) {
+            r_ln = sr * ((Real)1 - ur * ((Real)(1.0 / 3.0) + ur * (Real)(4.0 / 45.0)));
+            izg = iz * (c[0] + uz * (c[1] + uz * c[2]));
+        } else {
+            r_ln = sr * ((Real)1 - ur * ((Real)(1.0 / 3.0) + ur * ((Real)(4.0 / 45.0) + ur * (Real)(44.0 / 945.0))));
+            izg = iz * (c[0] + uz * (c[1] + uz * (c[2] + uz * c[3])));
+        }
         // wave-uniform branch (a divergent one makes the register allocator
         // keep both sides' values of every unrolled pair live); each lane then
         // selects its own value
