@@ -16,12 +16,10 @@ CPU oracle baseline are reported beside it.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
 
-import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -81,40 +79,6 @@ def time_form(D, torch, cfg, ic, steps, warmup, cfl, autotune):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
     return s, Q, ms, t, tuned
-
-
-def exact_vortex(cfg, t, x):
-    """Exact advected vortex at the nodes, summed over periodic images of the
-    Gaussian (the single-image form is not periodic at the 1e-6 level)."""
-    K, p = cfg.K, cfg.p
-    h = 10.0 / K
-    e = np.arange(K, dtype=np.float64)
-    Xb = h * (e[None, :, None, None] + 0.5 * (x[None, None, None, :] + 1.0))    # (1, K, 1, N)
-    Yb = h * (e[:, None, None, None] + 0.5 * (x[None, None, :, None] + 1.0))    # (K, 1, N, 1)
-    X = Xb + 0.0 * Yb
-    Y = Yb + 0.0 * Xb
-    g = 1.4; eps = 5.0
-    du = np.zeros(X.shape); dv = np.zeros(X.shape); dT = np.zeros(X.shape)
-    xs = (X - t) % 10.0; ys = (Y - t) % 10.0
-    for ix in (-10.0, 0.0, 10.0):
-        for iy in (-10.0, 0.0, 10.0):
-            dx = xs - 5.0 - ix; dy = ys - 5.0 - iy
-            r2 = dx * dx + dy * dy
-            f = eps / (2 * math.pi) * np.exp(0.5 * (1 - r2))
-            du += -f * dy; dv += f * dx
-            dT += -(g - 1) * eps * eps / (8 * g * math.pi ** 2) * np.exp(1 - r2)
-    T = 1 + dT
-    r = T ** (1 / (g - 1)); pr = T ** (g / (g - 1)); u = 1 + du; v = 1 + dv
-    return np.stack([r, r * u, r * v, pr / (g - 1) + 0.5 * r * (u * u + v * v)])
-
-
-def errors(torch, A, B, wq):
-    """GLL-weighted relative L2 and relative Linf of A vs B (device tensors)."""
-    A = A.view(4, -1); B = B.view(4, -1)
-    d = A - B
-    num = torch.sqrt(torch.sum(wq * d * d)).item()
-    den = torch.sqrt(torch.sum(wq * B * B)).item()
-    return num / den, (d.abs().max() / B.abs().max()).item()
 
 
 def cpu_baseline(cfg, ic, steps, cfl):
@@ -182,12 +146,9 @@ def bench_kh(args):
         Qs[form] = Q
         log(f"[bench-kh] {form}: {out[form]}")
     if "fp64" in Qs and "mixed" in Qs:
-        x, w, _ = s.operators()
-        h = 2.0 / K
-        wq = torch.from_numpy(np.broadcast_to(0.25 * h * h * np.outer(w, w), (K * K, N, N)).reshape(-1).copy()).cuda()
-        l2, linf = errors(torch, Qs["mixed"], Qs["fp64"], wq)
-        out["mixed_vs_fp64_rel_l2"] = l2
-        out["mixed_vs_fp64_rel_linf"] = linf
+        nrm = s.error_norms(Qs["mixed"], Qs["fp64"])
+        out["mixed_vs_fp64_rel_l2"] = nrm["rel_l2"]
+        out["mixed_vs_fp64_rel_linf"] = nrm["rel_linf"]
     print(json.dumps({"workload": f"Kelvin-Helmholtz {K}x{K} p={p} to t={args.t_end}, CFL {args.cfl}, "
                                   f"Chandrashekar + HLLC, AV + limiter {'off' if args.no_shock_capturing else 'on'}",
                       "unit": "DOF*stage/s", "dof": dof, **out}))
@@ -282,20 +243,19 @@ def main():
     acc = {}
     if not args.no_accuracy and "fp64" in sols and "mixed" in sols:
         s64, Q64, cfg64 = sols["fp64"]
-        x, w, _ = s64.operators()
-        h = 10.0 / K
-        wq = torch.from_numpy(np.broadcast_to(0.25 * h * h * np.outer(w, w), (K * K, N, N)).reshape(-1).copy()).cuda()
         s32, Q32, _ = sols["mixed"]
         t64 = res["fp64"]["t"]
         # equal-time comparison: run the mixed form again to exactly t64
         Q32 = s32.init_state(ic, Q32)
         s32.advance(Q32, 10 * args.steps, args.cfl, t_end=t64)
-        l2, linf = errors(torch, Q32, Q64, wq)
-        ex = torch.from_numpy(exact_vortex(cfg64, t64, x).reshape(4, -1)).cuda()
-        e64 = errors(torch, Q64, ex, wq)
-        e32 = errors(torch, Q32, ex, wq)
-        acc = {"mixed_vs_fp64_rel_l2": l2, "mixed_vs_fp64_rel_linf": linf,
-               "fp64_vs_exact_rel_l2": e64[0], "mixed_vs_exact_rel_l2": e32[0], "t": t64}
+        mf = s64.error_norms(Q32, Q64)
+        # exact advected vortex (periodic images summed), on the device
+        ex = s64.vortex_exact(ic, t64, images=True)
+        e64 = s64.error_norms(Q64, ex)
+        e32 = s64.error_norms(Q32, ex)
+        acc = {"mixed_vs_fp64_rel_l2": mf["rel_l2"], "mixed_vs_fp64_rel_linf": mf["rel_linf"],
+               "fp64_vs_exact_rel_l2": e64["rel_l2"], "mixed_vs_exact_rel_l2": e32["rel_l2"],
+               "fp64_vs_exact_l2_per_variable": e64["l2"], "t": t64}
         T = s64.totals(Q64)
         log(f"[bench] accuracy: {acc}")
 

@@ -105,7 +105,12 @@ def ensemble(n=256, seed=2024, precision="mixed", k_range=(16, 128), degrees=(3,
     """Seed-drawn isentropic-vortex instances: K ~ U{k_range}, p ~ U{degrees},
     strength ~ U[eps_range], background Mach ~ U[mach_range] along the diagonal
     (|(u0,v0)| = M*sqrt(gamma) for the unit background sound speed).
-    Returns (configs, ics, cfls); the CFL follows the degree's SSP-RK3 limit."""
+    Returns (configs, ics, cfls); the CFL follows the degree's SSP-RK3 limit
+    with a margin.  The dt formula's wave speed |u|+|v|+c is least
+    conservative at low Mach (the 2-D limit scales with (|u|+c)+(|v|+c)): a
+    weak, slow vortex grows an unstable mode within a few hundred steps at
+    CFL 0.5 (p=3), 0.3 (p=5) and 0.25 (p=7) in the FP64 oracle; 0.4, 0.25 and
+    0.2 stay stable over 1500 steps."""
     import numpy as np
     rng = np.random.default_rng(seed)
     cfgs, ics, cfls = [], [], []
@@ -117,5 +122,5 @@ def ensemble(n=256, seed=2024, precision="mixed", k_range=(16, 128), degrees=(3,
         speed = M * gamma ** 0.5 / 2 ** 0.5
         cfgs.append(vortex_config(K=K, p=p, precision=precision))
         ics.append(make_ic("vortex", vortex_eps=eps, u0=speed, v0=speed))
-        cfls.append(0.5 if p <= 3 else (0.35 if p <= 5 else 0.25))
+        cfls.append(0.4 if p <= 3 else (0.25 if p <= 5 else 0.2))
     return cfgs, ics, cfls

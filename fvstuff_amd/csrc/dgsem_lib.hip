@@ -2,6 +2,7 @@
 // are not per-degree: IC, unfused RK stage, wave-speed reductions, totals.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <cmath>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -25,7 +26,7 @@ struct dg_ctx {
     double* d_U2 = nullptr;
     DevState* d_st = nullptr;
     unsigned long long* d_red = nullptr; // reduction slot for dg_cfl_dt
-    double* d_part = nullptr;            // per-block partial sums for dg_totals
+    double* d_part = nullptr;            // per-block partials of dg_totals / dg_error_norms
     double* d_xi = nullptr;              // LGL node coordinates (IC kernel)
     int part_blocks = 0;
     int E = 0, xcd = 0;
@@ -144,6 +145,41 @@ __device__ __forceinline__ unsigned long long splitmix64(unsigned long long z) {
     return z ^ (z >> 31);
 }
 
+// Isentropic vortex (q1_dg_euler_2d.cu:2166-2188) at (x, y): strength eps,
+// centre (xc, yc), background (u0, v0).  images = 0: the single vortex (the
+// IC); images = 1: the velocity and temperature perturbations summed over the
+// 3x3 periodic images of the centre (period L), a periodic field.  Evaluated
+// without FMA contraction, like the IC.
+__device__ __forceinline__ void vortex_state(double x, double y, const ICArgs& a, int images, double& r,
+                                             double& u, double& v, double& p) {
+#pragma clang fp contract(off)
+    const double g = a.gamma, pi = 3.14159265358979323846;
+    double T;
+    if (!images) {
+        const double dx = x - a.xc, dy = y - a.yc, r2 = dx * dx + dy * dy;
+        const double f = a.eps / (2.0 * pi) * exp(0.5 * (1.0 - r2));
+        u = a.u0 - f * dy;
+        v = a.v0 + f * dx;
+        T = 1.0 + (-(g - 1.0) * a.eps * a.eps / (8.0 * g * pi * pi) * exp(1.0 - r2));
+    } else {
+        double du = 0.0, dv = 0.0, dT = 0.0;
+        for (int iy = -1; iy <= 1; iy++)
+            for (int ix = -1; ix <= 1; ix++) {
+                const double dx = x - (a.xc + ix * a.L), dy = y - (a.yc + iy * a.L), r2 = dx * dx + dy * dy;
+                const double f = a.eps / (2.0 * pi) * exp(0.5 * (1.0 - r2));
+                du += -f * dy;
+                dv += f * dx;
+                dT += -(g - 1.0) * a.eps * a.eps / (8.0 * g * pi * pi) * exp(1.0 - r2);
+            }
+        u = a.u0 + du;
+        v = a.v0 + dv;
+        T = 1.0 + dT;
+    }
+    if (T < 1e-6) T = 1e-6;
+    r = pow(T, 1.0 / (g - 1.0));
+    p = pow(T, g / (g - 1.0));
+}
+
 // IC at the LGL nodes (ic_kernel :692-717; vortex q1_dg_euler_2d.cu:2166-2188;
 // KH step layer with counter-based noise keyed by (seed, node)).  Coordinates
 // and the branch tests are evaluated without FMA contraction so that a node on
@@ -165,14 +201,7 @@ __global__ void ic_kernel(double* Q, ICArgs a) {
             const double phi = 0.5 * (1.0 + tanh((0.25 * a.L - rr) / (0.5 * a.h)));
             r = 0.125 + (a.rho_hi - 0.125) * phi; u = 0.0; v = 0.0; p = 0.1 + (a.p_hi - 0.1) * phi;
         } else if (a.kind == DG_IC_VORTEX) {
-            const double dx = x - a.xc, dy = y - a.yc, r2 = dx * dx + dy * dy;
-            const double f = a.eps / (2.0 * pi) * exp(0.5 * (1.0 - r2));
-            u = a.u0 - f * dy;
-            v = a.v0 + f * dx;
-            double T = 1.0 + (-(g - 1.0) * a.eps * a.eps / (8.0 * g * pi * pi) * exp(1.0 - r2));
-            if (T < 1e-6) T = 1e-6;
-            r = pow(T, 1.0 / (g - 1.0));
-            p = pow(T, g / (g - 1.0));
+            vortex_state(x, y, a, 0, r, u, v, p);
         } else {
             const unsigned long long z = splitmix64(a.seed + ((unsigned long long)c + 1ull) * 0x9e3779b97f4a7c15ull);
             const double uni = (double)(z >> 11) * (1.0 / 9007199254740992.0);
@@ -285,6 +314,96 @@ __global__ void totals_kernel(const double* Q, const double* w, int N, size_t S,
             for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += sred[q][i];
             part[(size_t)blockIdx.x * 5 + q] = s;
         }
+}
+
+// Exact vortex advected for time t by the background velocity: the vortex
+// field at the node coordinates shifted by -(u0, v0) t and wrapped into the
+// periodic domain [x0, x0+L) x [y0, y0+L) (the vortex is a steady solution
+// in the frame moving with the background).
+__global__ void vortex_exact_kernel(double* Q, ICArgs a, double t, int images) {
+#pragma clang fp contract(off)
+    const int N = a.N, N2 = N * N, NEL = a.K * a.K;
+    const size_t S = (size_t)NEL * N2;
+    const double g = a.gamma;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < S; c += (size_t)blockDim.x * gridDim.x) {
+        const int el = (int)(c / N2), nd = (int)(c - (size_t)el * N2);
+        const int ej = el / a.K, ei = el - ej * a.K, jn = nd / N, in = nd - jn * N;
+        double x = a.x0 + a.h * ((double)ei + 0.5 * (a.xi[in] + 1.0));
+        double y = a.y0 + a.h * ((double)ej + 0.5 * (a.xi[jn] + 1.0));
+        if (t != 0.0) {
+            x -= a.u0 * t;
+            y -= a.v0 * t;
+            x = a.x0 + (x - a.x0) - a.L * floor((x - a.x0) / a.L);
+            y = a.y0 + (y - a.y0) - a.L * floor((y - a.y0) / a.L);
+        }
+        double r, u, v, p;
+        vortex_state(x, y, a, images, r, u, v, p);
+        Q[c] = r;
+        Q[S + c] = r * u;
+        Q[2 * S + c] = r * v;
+        Q[3 * S + c] = p / (g - 1.0) + 0.5 * r * (u * u + v * v);
+    }
+}
+
+// Per-block partials of the GLL-weighted norms of Q - R: sums of w d_q^2 (4),
+// max |d_q| (4), sum of w R_q^2 over q, max |R_q| over q.
+constexpr int NORM_PARTS = 10;
+__global__ void error_norms_kernel(const double* Q, const double* R, const double* w, int N, size_t S, double jw,
+                                   double* part) {
+    __shared__ double sred[NORM_PARTS][BLOCK / 64];
+    double acc[NORM_PARTS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int N2 = N * N;
+    for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < S; n += (size_t)blockDim.x * gridDim.x) {
+        const int nd = (int)(n % N2), jn = nd / N, in = nd - jn * N;
+        const double wij = jw * w[in] * w[jn];
+        for (int q = 0; q < 4; q++) {
+            const double r = R[q * S + n], d = Q[q * S + n] - r;
+            acc[q] += wij * d * d;
+            acc[4 + q] = nanmax(acc[4 + q], fabs(d));
+            acc[8] += wij * r * r;
+            acc[9] = nanmax(acc[9], fabs(r));
+        }
+    }
+    for (int k = 0; k < NORM_PARTS; k++) {
+        double v = acc[k];
+        for (int o = 32; o > 0; o >>= 1) {
+            const double x = __shfl_xor(v, o, 64);
+            v = (k < 4 || k == 8) ? v + x : nanmax(v, x);
+        }
+        acc[k] = v;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < NORM_PARTS; k++) sred[k][wid] = acc[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NORM_PARTS; k++) {
+            double v = sred[k][0];
+            for (int i = 1; i < (int)(blockDim.x >> 6); i++)
+                v = (k < 4 || k == 8) ? v + sred[k][i] : nanmax(v, sred[k][i]);
+            part[(size_t)blockIdx.x * NORM_PARTS + k] = v;
+        }
+}
+
+// Element averages of rho and p for frame output (downsample_rho_p,
+// :826-844): plain node means, rho floored at 1e-14, negative pressures
+// counted as 0; nodes summed in storage order.  One thread per element.
+__global__ void frame_fields_kernel(const double* Q, int NEL, int N2, double g, double* rho_el, double* p_el) {
+    const size_t S = (size_t)NEL * N2;
+    for (int el = blockIdx.x * blockDim.x + threadIdx.x; el < NEL; el += blockDim.x * gridDim.x) {
+        double rs = 0.0, ps = 0.0;
+        for (int n = 0; n < N2; n++) {
+            const size_t i = (size_t)el * N2 + n;
+            double rho = Q[i];
+            const double mx = Q[S + i], my = Q[2 * S + i], E = Q[3 * S + i];
+            if (rho < 1e-14) rho = 1e-14;
+            const double pr = (g - 1.0) * (E - 0.5 * (mx * mx + my * my) / rho);
+            rs += rho;
+            ps += pr > 0.0 ? pr : 0.0;
+        }
+        rho_el[el] = rs / N2;
+        p_el[el] = ps / N2;
+    }
 }
 
 // ── dispatch tables ───────────────────────────────────────────────────────
@@ -502,7 +621,7 @@ int dg_create(const dg_config* cfg, void* stream, dg_ctx** out) {
     CK_CREATE(hipMalloc(&c->d_U2, 4 * S * sizeof(double)));
     CK_CREATE(hipMalloc(&c->d_st, sizeof(DevState)));
     CK_CREATE(hipMalloc(&c->d_red, sizeof(unsigned long long)));
-    CK_CREATE(hipMalloc(&c->d_part, (size_t)c->part_blocks * 5 * sizeof(double)));
+    CK_CREATE(hipMalloc(&c->d_part, (size_t)c->part_blocks * NORM_PARTS * sizeof(double)));
     CK_CREATE(hipMalloc(&c->d_xi, 16 * sizeof(double)));
     CK_CREATE(hipMemcpyAsync(c->d_xi, c->x, c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CK_CREATE(hipMemcpyAsync(c->d_tab64, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -516,15 +635,20 @@ int dg_create(const dg_config* cfg, void* stream, dg_ctx** out) {
     return DG_OK;
 }
 
-int dg_init_state(dg_ctx* c, const dg_ic_params* ic, double* Q) {
-    if (!c || !ic || !Q) return fail(c, DG_ERR_INVALID_ARG, "dg_init_state: null argument");
-    if (ic->kind < 0 || ic->kind > 2) return fail(c, DG_ERR_INVALID_ARG, "dg_init_state: unknown IC %d", ic->kind);
+static ICArgs ic_args(const dg_ctx* c, const dg_ic_params* ic) {
     ICArgs a;
     a.kind = ic->kind; a.K = c->cfg.K; a.N = c->N;
     a.x0 = c->cfg.x0; a.y0 = c->cfg.y0; a.L = c->cfg.L; a.h = c->h; a.gamma = c->cfg.gamma;
     a.rho_hi = ic->rho_hi; a.p_hi = ic->p_hi; a.eps = ic->vortex_eps; a.xc = ic->xc; a.yc = ic->yc;
     a.u0 = ic->u0; a.v0 = ic->v0; a.seed = ic->seed; a.amp = ic->noise_amp;
     a.xi = c->d_xi;
+    return a;
+}
+
+int dg_init_state(dg_ctx* c, const dg_ic_params* ic, double* Q) {
+    if (!c || !ic || !Q) return fail(c, DG_ERR_INVALID_ARG, "dg_init_state: null argument");
+    if (ic->kind < 0 || ic->kind > 2) return fail(c, DG_ERR_INVALID_ARG, "dg_init_state: unknown IC %d", ic->kind);
+    const ICArgs a = ic_args(c, ic);
     const size_t S = (size_t)c->NEL * c->N2;
     hipLaunchKernelGGL(ic_kernel, dim3(grid_for(S, BLOCK)), dim3(BLOCK), 0, c->stream, Q, a);
     HIPCK(c, hipGetLastError());
@@ -680,6 +804,56 @@ int dg_totals(dg_ctx* c, const double* Q, double out[5]) {
         for (int b = 0; b < c->part_blocks; b++) s += part[(size_t)b * 5 + q];
         out[q] = s;
     }
+    return DG_OK;
+}
+
+int dg_vortex_exact(dg_ctx* c, const dg_ic_params* ic, double t, int images, double* Q) {
+    if (!c || !ic || !Q) return fail(c, DG_ERR_INVALID_ARG, "dg_vortex_exact: null argument");
+    if (ic->kind != DG_IC_VORTEX) return fail(c, DG_ERR_INVALID_ARG, "dg_vortex_exact: IC kind %d is not the vortex", ic->kind);
+    if (!(t >= 0.0) || (images != 0 && images != 1)) return fail(c, DG_ERR_INVALID_ARG, "dg_vortex_exact: bad t or images");
+    const ICArgs a = ic_args(c, ic);
+    const size_t S = (size_t)c->NEL * c->N2;
+    hipLaunchKernelGGL(vortex_exact_kernel, dim3(grid_for(S, BLOCK)), dim3(BLOCK), 0, c->stream, Q, a, t, images);
+    HIPCK(c, hipGetLastError());
+    return DG_OK;
+}
+
+int dg_error_norms(dg_ctx* c, const double* Q, const double* Qref, double out[10]) {
+    if (!c || !Q || !Qref || !out) return fail(c, DG_ERR_INVALID_ARG, "dg_error_norms: null argument");
+    const size_t S = (size_t)c->NEL * c->N2;
+    hipLaunchKernelGGL(error_norms_kernel, dim3(c->part_blocks), dim3(BLOCK), 0, c->stream, Q, Qref,
+                       (const double*)c->d_tab64 + c->N2, c->N, S, 0.25 * c->h * c->h, c->d_part);
+    HIPCK(c, hipGetLastError());
+    std::vector<double> part((size_t)c->part_blocks * NORM_PARTS);
+    HIPCK(c, hipMemcpyAsync(part.data(), c->d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    double tot[NORM_PARTS];
+    for (int k = 0; k < NORM_PARTS; k++) {
+        double v = part[k];
+        for (int b = 1; b < c->part_blocks; b++) {
+            const double x = part[(size_t)b * NORM_PARTS + k];
+            if (k < 4 || k == 8) v += x;
+            else v = (v != v || x != x) ? v + x : std::max(v, x);
+        }
+        tot[k] = v;
+    }
+    double sum = 0.0, mx = 0.0;
+    for (int q = 0; q < 4; q++) {
+        out[q] = std::sqrt(tot[q]);
+        out[4 + q] = tot[4 + q];
+        sum += tot[q];
+        mx = (mx != mx || tot[4 + q] != tot[4 + q]) ? mx + tot[4 + q] : std::max(mx, tot[4 + q]);
+    }
+    out[8] = std::sqrt(sum) / std::sqrt(tot[8]);
+    out[9] = mx / tot[9];
+    return DG_OK;
+}
+
+int dg_frame_fields(dg_ctx* c, const double* Q, double* rho_el, double* p_el) {
+    if (!c || !Q || !rho_el || !p_el) return fail(c, DG_ERR_INVALID_ARG, "dg_frame_fields: null argument");
+    hipLaunchKernelGGL(frame_fields_kernel, dim3(grid_for(c->NEL, 256)), dim3(256), 0, c->stream, Q, c->NEL, c->N2,
+                       c->cfg.gamma, rho_el, p_el);
+    HIPCK(c, hipGetLastError());
     return DG_OK;
 }
 

@@ -12,6 +12,7 @@ Q[var, el, jn, in] flattened); the numerics run in libdgsem.so.  There is no
 CPU fallback: a missing library or a missing GPU raises.
 """
 import ctypes as C
+import math
 import os
 
 import torch
@@ -47,6 +48,9 @@ SIGNATURES = {
     "dg_set_graph": (C.c_int, [_vp, C.c_int]),
     "dg_sensor": (C.c_int, [_vp, _vp, _vp]),
     "dg_totals": (C.c_int, [_vp, _vp, _dp]),
+    "dg_vortex_exact": (C.c_int, [_vp, C.POINTER(DGICParams), C.c_double, C.c_int, _vp]),
+    "dg_error_norms": (C.c_int, [_vp, _vp, _vp, _dp]),
+    "dg_frame_fields": (C.c_int, [_vp, _vp, _vp, _vp]),
     "dg_autotune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "dg_batch_create": (C.c_int, [C.POINTER(DGConfig), C.c_int, _vp, C.POINTER(_vp)]),
     "dg_batch_destroy": (C.c_int, [_vp]),
@@ -186,11 +190,16 @@ class Solver:
         chunk); returns (t, steps taken).  progress(t, steps) is called after
         every chunk if given."""
         steps = 0
+        n = min(chunk, 8)            # a short first chunk measures dt
         while t < t_end - 1e-12 * max(1.0, abs(t_end)) and steps < max_steps:
-            t, _ = self.advance(Q, chunk, cfl, t_end=t_end, t=t)
+            t, dt = self.advance(Q, n, cfl, t_end=t_end, t=t)
             steps += self.steps_taken()
             if progress is not None:
                 progress(t, steps)
+            # steps past t_end are launched but skipped on the device: size the
+            # next chunk from the last dt (dt varies slowly), with a margin
+            if dt > 0.0:
+                n = int(min(chunk, max(2, math.ceil(1.02 * (t_end - t) / dt) + 2)))
         return t, steps
 
     def step(self, Q, cfl):
@@ -223,6 +232,26 @@ class Solver:
         out = (C.c_double * 5)()
         self._check(self.lib.dg_totals(self._ctx, _ptr(Q), out))
         return list(out)
+
+    def vortex_exact(self, ic, t, images=True, Q=None):
+        """Exact advected vortex at time t (periodic images summed by default)."""
+        Q = self.new_state() if Q is None else Q
+        self._check(self.lib.dg_vortex_exact(self._ctx, C.byref(ic), float(t), int(bool(images)), _ptr(Q)))
+        return Q
+
+    def error_norms(self, Q, Qref):
+        """GLL-quadrature norms of Q - Qref: dict with per-variable L2 and max
+        errors and the relative L2 / Linf over all variables."""
+        out = (C.c_double * 10)()
+        self._check(self.lib.dg_error_norms(self._ctx, _ptr(Q), _ptr(Qref), out))
+        v = list(out)
+        return {"l2": v[0:4], "max": v[4:8], "rel_l2": v[8], "rel_linf": v[9]}
+
+    def frame_fields(self, Q):
+        """Per-element mean rho and p (device tensors of K*K), the data of a frame."""
+        rho, p = self.new_elem(), self.new_elem()
+        self._check(self.lib.dg_frame_fields(self._ctx, _ptr(Q), _ptr(rho), _ptr(p)))
+        return rho, p
 
     def autotune(self):
         E = C.c_int(); X = C.c_int()

@@ -144,7 +144,10 @@ int         dg_cfl_dt(dg_ctx* ctx, const double* lam_el, double cfl,
  * computed on the device from the wave speed of the state at the start of the
  * step, dt = min(cfl*h/(lam_max*(p+1)), t_end - t) (the reference's loop,
  * :1022-1058, with frame clipping replaced by one end time; pass t_end <= 0
- * for no clipping).  No host synchronisation happens until the end.
+ * for no clipping).  No host synchronisation happens until the end.  Steps
+ * after t_end is reached are still launched and end early on the device
+ * (each still streams the state once), so size nsteps from the last dt when
+ * advancing to an end time.
  * t_inout: simulation time in/out (may be NULL); dt_last: last dt (may be NULL).
  * Returns the first numerical failure (NONFINITE / BLOWUP / DT_UNDERFLOW), in
  * which case the state is left as it was after the last good step. */
@@ -170,6 +173,33 @@ int         dg_sensor(dg_ctx* ctx, const double* Q, double* sensor_el);
  * out[0..3] = integrals of rho, rho*u, rho*v, E;  out[4] = integral of the
  * entropy -rho*s/(gamma-1), s = ln(p*rho^-gamma). */
 int         dg_totals(dg_ctx* ctx, const double* Q, double out[5]);
+
+/* Exact isentropic vortex (vortex_paper_exact, q1_dg_euler_2d.cu:2166-2188,
+ * the reference solution of its L2 error kernel :2247-2294) advected for
+ * time t >= 0 by the background velocity (u0, v0) on the periodic domain:
+ * the vortex field at the node coordinates shifted by -(u0, v0) t and wrapped
+ * into the domain, written to Q (SIDX).  images = 0: the single vortex (at
+ * t = 0 identical to dg_init_state); images = 1: velocity and temperature
+ * perturbations summed over the 3x3 periodic images of the centre, a
+ * periodic field (the single vortex is periodic only to ~1e-5 at the edges).
+ * ic->kind must be DG_IC_VORTEX. */
+int         dg_vortex_exact(dg_ctx* ctx, const dg_ic_params* ic, double t, int images, double* Q);
+
+/* Norms of Q - Qref with the GLL quadrature of the discretisation, FP64 on
+ * the device, deterministic, copied to host (the per-variable L2 errors of
+ * compute_l2_err_paper, q1_dg_euler_2d.cu:2247-2294):
+ * out[0..3] = sqrt(integral of (Q_q - Qref_q)^2) for rho, rho*u, rho*v, E;
+ * out[4..7] = max |Q_q - Qref_q|;  out[8] = relative L2 over all four
+ * variables, sqrt(sum_q int d_q^2) / sqrt(sum_q int Qref_q^2);
+ * out[9] = max_q max|d_q| / max_q max|Qref_q|.  A NaN propagates. */
+int         dg_error_norms(dg_ctx* ctx, const double* Q, const double* Qref, double out[10]);
+
+/* Frame fields (downsample_rho_p, dgsem_lobatto_2d.cu:826-844, the data of
+ * the reference's PPM frames and of plot_rt.py's binary frames): per element,
+ * the mean over its nodes of rho (floored at 1e-14) and of max(p, 0).
+ * rho_el, p_el: device arrays of K*K doubles, element order el = ej*K + ei
+ * (row-major, j = 0 at the bottom).  FP64, deterministic. */
+int         dg_frame_fields(dg_ctx* ctx, const double* Q, double* rho_el, double* p_el);
 
 /* Tuning: time a few candidate schedules (elements per block x XCD remap)
  * on scratch buffers and keep the fastest for this ctx.  Optional; dg_create

@@ -793,6 +793,32 @@ OR_API int oracle_sensor(void* ctx, const double* Q, double* sensor)
     return DG_OK;
 }
 
+/* Frame fields, downsample_rho_p (dgsem_lobatto_2d.cu:826-844) in FP64:
+ * per element the plain mean over its nodes (storage order) of rho, floored
+ * at 1e-14, and of the pressure clipped below at 0. */
+OR_API int oracle_frame_fields(void* ctx, const double* Q, double* rho_el, double* p_el)
+{
+    Ctx& c = *(Ctx*)ctx;
+    const double g = c.cfg.gamma;
+    const int nn = c.NN;
+    for (int el = 0; el < c.NEL; el++) {
+        double rs = 0.0, ps = 0.0;
+        for (int n = 0; n < c.NN2; n++) {
+            double rho = Q[sidx(c, 0, el, n / nn, n % nn)];
+            double mx = Q[sidx(c, 1, el, n / nn, n % nn)];
+            double my = Q[sidx(c, 2, el, n / nn, n % nn)];
+            double E = Q[sidx(c, 3, el, n / nn, n % nn)];
+            if (rho < 1e-14) rho = 1e-14;
+            double p = (g - 1.0) * (E - 0.5 * (mx * mx + my * my) / rho);
+            rs += rho;
+            ps += (p > 0.0) ? p : 0.0;
+        }
+        rho_el[el] = rs / c.NN2;
+        p_el[el] = ps / c.NN2;
+    }
+    return DG_OK;
+}
+
 /* GLL totals (pattern of q1_dg_euler_2d.cu:3004-3016, on the nodal state):
  * out[0..3] conserved integrals, out[4] entropy integral -rho s/(g-1). */
 OR_API int oracle_totals(void* ctx, const double* Q, double* out)

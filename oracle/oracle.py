@@ -51,6 +51,7 @@ def lib():
         L.oracle_advance.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, _dp, _dp]
         L.oracle_sensor.argtypes = [C.c_void_p, _dp, _dp]
         L.oracle_totals.argtypes = [C.c_void_p, _dp, _dp]
+        L.oracle_frame_fields.argtypes = [C.c_void_p, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -176,6 +177,11 @@ class Oracle:
         s = np.zeros(self.NEL)
         lib().oracle_sensor(self._ctx, _p(Q), _p(s))
         return s
+
+    def frame_fields(self, Q):
+        rho = np.zeros(self.NEL); p = np.zeros(self.NEL)
+        lib().oracle_frame_fields(self._ctx, _p(Q), _p(rho), _p(p))
+        return rho, p
 
     def totals(self, Q):
         out = np.zeros(5)

@@ -17,7 +17,7 @@ def test_ensemble_ranges_and_determinism():
         assert 1.0 <= ic.vortex_eps <= 5.0
         mach = math.hypot(ic.u0, ic.v0) / math.sqrt(1.4)
         assert 0.1 - 1e-12 <= mach <= 0.8 + 1e-12
-        assert cfl == (0.5 if c.p == 3 else (0.35 if c.p == 5 else 0.25))
+        assert cfl == (0.4 if c.p == 3 else (0.25 if c.p == 5 else 0.2))
     cfgs2, ics2, _ = ensemble(n=256, seed=2024)
     assert [(c.K, c.p) for c in cfgs] == [(c.K, c.p) for c in cfgs2]
     assert [i.vortex_eps for i in ics] == [i.vortex_eps for i in ics2]

@@ -55,9 +55,10 @@ RHS_CASES = [
 
 
 def stable_cfl(p):
-    # SSP-RK3 with dt = cfl*h/(lam*(p+1)), Ranocha + Rusanov: the oracle is
-    # stable up to about 0.6 at p=3, 0.4 at p=5 and 0.3 at p=7
-    return 0.5 if p <= 3 else (0.35 if p <= 5 else 0.25)
+    # SSP-RK3 with dt = cfl*h/(lam*(p+1)), Ranocha + Rusanov: on the vortex
+    # the oracle is stable over hundreds of steps up to about 0.5 at p=3, 0.3
+    # at p=5 and 0.2 at p=7 (0.35 / 0.25 grow an unstable mode for some draws)
+    return 0.5 if p <= 3 else (0.3 if p <= 5 else 0.2)
 
 
 def make(kind_kwargs, precision="fp64"):
