@@ -333,11 +333,11 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     Real* sW = (Real*)smem;                          // [E][N2][5] prims; reused as sR [E][4][N2]
     Real* sR = sW;
     double* sU = (double*)(smem + a.off_U);         // [E][4][N2] FP64 stage input, then stage values
-    Real* sUc = (Real*)(smem + a.off_av);           // [E][4][N2] AV: sanitized cons
-    Real* sDu = sUc + (size_t)E * 4 * N2;           // [E][N2]
-    Real* sDv = sDu + (size_t)E * N2;               // [E][N2]
-    Real* sNu = sDv + (size_t)E * N2;               // [E]
-    double* sLam = (double*)(smem + a.off_lam);     // [E][N2]
+    Real* sDu = (Real*)(smem + a.off_av);           // [E][N2] AV: du/dx
+    Real* sDv = sDu + (size_t)E * N2;               // [E][N2] AV: dv/dy
+    Real* sNu = sDv + (size_t)E * N2;               // [E]     AV: element viscosity
+    unsigned long long* sLamE = (unsigned long long*)(smem + a.off_lam);   // [E] max wave speed (bits)
+    unsigned long long* sThE = sLamE + E;           // [E] max sensor (bits of the double)
     double* sEl = (double*)(smem + a.off_el);       // per-element scratch
     double* sBar = sEl;                              // [E][4]
     double* sTrho = sEl + 4 * E;                     // [E]
@@ -465,6 +465,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         dt = s_dt;
     }
 
+    if (FULL && (need_lam || a.av)) {
+        // element maxima gathered with LDS atomics on the ordered bit patterns
+        // of non-negative doubles (order-independent, so deterministic)
+        if (tid < E) { sLamE[tid] = 0ull; sThE[tid] = 0ull; }
+        __syncthreads();
+    }
 #pragma unroll
     for (int j = 0; j < JMAX; j++) {
         const int s = tid + j * nthr;
@@ -482,14 +488,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         Real* p = sW + (size_t)s * 5;
         p[0] = (Real)0.5 * w.r; p[1] = (Real)0.5 * w.u; p[2] = (Real)0.5 * w.v;
         p[3] = (Real)0.5 * w.p; p[4] = (Real)0.5 * w.z;
-        if (FULL) {
-            if (need_lam) sLam[s] = Cons64::wave_speed(W, g);
-            if (a.av) {
-                Real* c = sUc + (size_t)e * 4 * N2 + nd;
-                c[0] = w.r; c[N2] = w.r * w.u; c[2 * N2] = w.r * w.v;
-                c[3 * N2] = w.p * ph.inv_gm1 + (Real)0.5 * w.r * (w.u * w.u + w.v * w.v);
-            }
-        }
+        if (FULL && need_lam && s < nE * N2) atomicMax(&sLamE[e], dbits(Cons64::wave_speed(W, g)));
     }
     __syncthreads();
 
@@ -510,22 +509,23 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             }
         }
         __syncthreads();
-        if (tid < E) {
-            const int e = tid;
-            const Real len_p = (Real)(a.h / (2 * P + 1));
-            const Real dx2inv = (Real)1 / (len_p * len_p);
-            Real th_e = 0;
-            double lam_e = 0.0;
-            for (int nd = 0; nd < N2; nd++) {
-                const Real* p = sW + ((size_t)e * N2 + nd) * 5;
-                Real rho = Mx::fmax_((Real)2 * p[0], (Real)1e-12), c2 = ph.g * ((Real)2 * p[3]) / rho;
-                Real divu = Mx::fmin_(jac * (sDu[e * N2 + nd] + sDv[e * N2 + nd]), (Real)0), du2 = divu * divu;
-                Real th = du2 / (du2 + (Real)a.av_k * c2 * dx2inv + (Real)1e-30);
-                th_e = Mx::fmax_(th_e, th);
-                lam_e = fmax(lam_e, sLam[e * N2 + nd]);
-            }
-            sNu[e] = (Real)a.av_cav * th_e * len_p * (Real)lam_e;
+        // Ducros-type sensor per node, max per element (:488-491)
+        const Real len_p = (Real)(a.h / (2 * P + 1));
+        const Real dx2inv = (Real)1 / (len_p * len_p);
+#pragma unroll
+        for (int j = 0; j < JMAX; j++) {
+            const int s = tid + j * nthr;
+            if (s >= nE * N2) continue;
+            const int e = s / N2;
+            const Real* p = sW + (size_t)s * 5;
+            Real rho = Mx::fmax_((Real)2 * p[0], (Real)1e-12), c2 = ph.g * ((Real)2 * p[3]) / rho;
+            Real divu = Mx::fmin_(jac * (sDu[s] + sDv[s]), (Real)0), du2 = divu * divu;
+            Real th = du2 / (du2 + (Real)a.av_k * c2 * dx2inv + (Real)1e-30);
+            atomicMax(&sThE[e], dbits((double)th));
         }
+        __syncthreads();
+        if (tid < E)
+            sNu[tid] = (Real)a.av_cav * (Real)bitsd(sThE[tid]) * len_p * (Real)bitsd(sLamE[tid]);
         __syncthreads();
     }
 
@@ -603,14 +603,14 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         if (FULL && a.av) {
             const Real nu = sNu[le];
             const Real* wq = D + N * N;
-            Real uc[N][4];
+            Real uc[N][4];      // conserved variables of the line's nodes (u, v as rotated)
 #pragma unroll
             for (int k = 0; k < N; k++) {
-                const Real* c = sUc + (size_t)le * 4 * N2 + lbase + k * lstride;
-                uc[k][0] = c[0];
-                uc[k][1] = ydir ? c[2 * N2] : c[N2];
-                uc[k][2] = ydir ? c[N2] : c[2 * N2];
-                uc[k][3] = c[3 * N2];
+                const Prim<Real> w = doubled(hw[k]);
+                uc[k][0] = w.r;
+                uc[k][1] = w.r * w.u;
+                uc[k][2] = w.r * w.v;
+                uc[k][3] = w.p * ph.inv_gm1 + (Real)0.5 * w.r * (w.u * w.u + w.v * w.v);
             }
             Real gr[N][4];
 #pragma unroll
@@ -681,11 +681,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
 #pragma unroll
             for (int q = 0; q < 4; q++) a.out[q * S + o] = (double)r[q * N2];
         }
-        if (FULL && a.lam_el != nullptr && tid < nE) {
-            double lam_e = 0.0;
-            for (int nd = 0; nd < N2; nd++) lam_e = fmax(lam_e, sLam[tid * N2 + nd]);
-            a.lam_el[el0 + tid] = lam_e;
-        }
+        if (FULL && a.lam_el != nullptr && tid < nE) a.lam_el[el0 + tid] = bitsd(sLamE[tid]);
         return;
     }
 
@@ -789,9 +785,9 @@ inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, b
     L.off_U = off;
     if (stage) off += align_up(E * 4 * N2 * 8, 16);
     L.off_av = off;
-    if (av) off += align_up((E * 4 * N2 + 2 * E * N2 + E) * real_bytes, 16);
+    if (av) off += align_up((2 * E * N2 + E) * real_bytes, 16);
     L.off_lam = off;
-    if (lam || av) off += align_up(E * N2 * 8, 16);
+    if (lam || av) off += align_up(2 * E * 8, 16);
     L.off_el = off;
     off += align_up((6 * E + BLOCK / 64) * 8, 16);
     L.total = off;
