@@ -157,7 +157,7 @@ def bench_kh(args):
 def bench_batched(args):
     """Seeded ensemble of isentropic-vortex instances (K ~ U{16..128},
     p ~ U{3,5,7}, strength ~ U[1,5], Mach ~ U[0.1,0.8]) stepped in one
-    persistent launch per stage vs instance by instance."""
+    launch per stage (one workgroup per tile) vs instance by instance."""
     import torch
     from fvstuff_amd import dgsem as D
     from fvstuff_amd.capi import ensemble
@@ -165,7 +165,7 @@ def bench_batched(args):
     cfgs, ics, cfls = ensemble(n=args.instances, seed=args.seed, precision=form)
     b = D.Batch(cfgs)
     Qs = b.init_states(ics)
-    grid = b.autotune(Qs, cfls) if not args.no_autotune else None
+    lanes = b.autotune(Qs, cfls) if not args.no_autotune else None
     res = {}
     for mode in ("batched", "each"):
         Qs = b.init_states(ics, Qs)
@@ -180,7 +180,7 @@ def bench_batched(args):
         res[mode] = {"value": b.dofs * 3 * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps}
         log(f"[bench-batched] {mode}: {res[mode]}")
     print(json.dumps({"workload": f"{args.instances} vortex instances (seed {args.seed}), {form}",
-                      "unit": "DOF*stage/s", "dof": b.dofs, "steps": args.steps, "grid": grid,
+                      "unit": "DOF*stage/s", "dof": b.dofs, "steps": args.steps, "lanes_per_workgroup": lanes,
                       "batched": res["batched"], "one_by_one": res["each"],
                       "speedup": res["batched"]["value"] / res["each"]["value"]}))
 

@@ -1,5 +1,5 @@
-// Batched (multi-instance) kernels: the persistent stage kernel over a tile
-// list, plus the per-tile wave-speed kernel that seeds each instance's dt.
+// Batched (multi-instance) stage launches: one workgroup per tile of the
+// tile list, every instance and degree in the same launch.
 // Compiled once per compute precision (-DDG_REAL=float | double).
 #include "dgsem_launch.hpp"
 
@@ -19,7 +19,7 @@ namespace dg {
             if (e != hipSuccess) return e;                                                               \
             done = lds;                                                                                  \
         }                                                                                                \
-        hipLaunchKernelGGL(k, dim3(grid), dim3(BLOCK), lds, s, args, tiles, ntiles, stage, sc);          \
+        hipLaunchKernelGGL(k, dim3(ntiles), dim3(lanes), lds, s, args, tiles, ntiles, stage, sc);          \
         return hipGetLastError();                                                                        \
     }
 
@@ -27,23 +27,12 @@ namespace dg {
 #define DG_CAT(a, b) DG_CAT2(a, b)
 
 hipError_t DG_CAT(launch_batch_stage_, DG_REAL)(const KArgs* args, const int2* tiles, int ntiles, int stage,
-                                                StepCtl sc, int vol, int grid, size_t lds, hipStream_t s) {
+                                                StepCtl sc, int vol, int lanes, size_t lds, hipStream_t s) {
     switch (vol) {
         case VOL_RANOCHA: DG_BATCH_LAUNCH(VOL_RANOCHA)
         case VOL_CHANDRASHEKAR: DG_BATCH_LAUNCH(VOL_CHANDRASHEKAR)
         default: DG_BATCH_LAUNCH(VOL_COLLOCATION)
     }
-}
-
-hipError_t DG_CAT(batch_stage_occupancy_, DG_REAL)(int vol, size_t lds, int* blocks_per_cu) {
-    const void* k = vol == VOL_RANOCHA ? (const void*)batch_stage_kernel<DG_REAL, false, VOL_RANOCHA>
-                  : vol == VOL_CHANDRASHEKAR ? (const void*)batch_stage_kernel<DG_REAL, false, VOL_CHANDRASHEKAR>
-                                             : (const void*)batch_stage_kernel<DG_REAL, false, VOL_COLLOCATION>;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, BLOCK, lds);
 }
 
 }  // namespace dg

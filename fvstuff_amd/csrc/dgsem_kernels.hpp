@@ -202,17 +202,18 @@ __device__ __forceinline__ void rot(Prim<Real>& w, bool swap) {
     w.v = swap ? u : v;
 }
 
-// FP64: materialise a pair's two accumulator rows where the pair is
-// evaluated (an empty asm that reads and writes them, no instruction).
-// Without it the compiler sinks every pair's flux and accumulation past the
-// later pairs' log-branches into one block at the end of the line, keeping
-// all pairs' intermediate values live at once: at p = 7 that needs the whole
-// register file (1 wave per SIMD) instead of 218 registers (2 waves).  The
-// FP32 form keeps the sunk schedule, whose wide independent block runs
-// faster at its occupancy (measured).
-template <int N, typename Real>
+// PIN: materialise a pair's two accumulator rows where the pair is
+// evaluated (an empty asm that reads and writes them, no instruction; the
+// arithmetic and its results are unchanged).  Without it the compiler sinks
+// every pair's flux and accumulation past the later pairs' log-branches into
+// one block at the end of the line, keeping all pairs' intermediate values
+// live at once: in FP64 at p = 7 that needs the whole register file (1 wave
+// per SIMD) instead of 218 registers (2 waves).  The FP32 single-solve kernel
+// keeps the sunk schedule, whose wide independent block runs faster at its
+// occupancy (measured); the batched kernel pins (see batch_stage_kernel).
+template <int N, bool PIN, typename Real>
 __device__ __forceinline__ void pin_pair(Real (&acc)[N][4], int i, int k) {
-    if (sizeof(Real) != 8) return;
+    if (!PIN) return;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         asm volatile("" : "+v"(acc[i][q]));
@@ -221,7 +222,7 @@ __device__ __forceinline__ void pin_pair(Real (&acc)[N][4], int i, int k) {
 }
 
 // ── symmetric pair loop of one line: F#(U_i,U_k) once per i < k ──────────
-template <int N, typename Real, int VOL>
+template <int N, typename Real, int VOL, bool PIN>
 __device__ __forceinline__ void line_pairs(const Phys<Real>& ph, const Prim<Real> (&hw)[N], const Real* D,
                                            Real (&acc)[N][4]) {
     if (VOL == VOL_RANOCHA) {
@@ -239,7 +240,7 @@ __device__ __forceinline__ void line_pairs(const Phys<Real>& ph, const Prim<Real
                     acc[i][q] += dik * F[q];
                     acc[k][q] += dki * F[q];
                 }
-                pin_pair<N>(acc, i, k);
+                pin_pair<N, PIN>(acc, i, k);
             }
         }
         return;
@@ -256,13 +257,13 @@ __device__ __forceinline__ void line_pairs(const Phys<Real>& ph, const Prim<Real
                 acc[i][q] += dik * F[q];
                 acc[k][q] += dki * F[q];
             }
-            pin_pair<N>(acc, i, k);
+            pin_pair<N, PIN>(acc, i, k);
         }
     }
 }
 
 // ── volume contribution of one line: acc[i] = sum_k D_ik F#(U_i,U_k) ──────
-template <int N, typename Real, int VOL>
+template <int N, typename Real, int VOL, bool PIN>
 __device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Real> (&hw)[N], const Real* D,
                                             Real (&acc)[N][4]) {
     using Mx = M<Real>;
@@ -292,7 +293,7 @@ __device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Rea
 #pragma unroll
         for (int q = 0; q < 4; q++) acc[i][q] += dii * f[q];
     }
-    line_pairs<N, Real, VOL>(ph, hw, D, acc);
+    line_pairs<N, Real, VOL, PIN>(ph, hw, D, acc);
 }
 
 // Register budget: a line keeps its p+1 node states and 4(p+1) accumulators
@@ -310,7 +311,7 @@ template <int P, int VOL, bool FULL> struct Occ<P, double, VOL, FULL> {
 // One tile (E consecutive elements) of one stage.  `lead` marks the one tile
 // per launch that updates the device step state.  Every early exit is taken
 // by the whole block (decided by thread 0, broadcast through LDS).
-template <int P, typename Real, bool FULL, int VOL>
+template <int P, typename Real, bool FULL, int VOL, bool PIN = (sizeof(Real) == 8)>
 __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int tile, bool lead) {
     constexpr int N = P + 1, N2 = N * N;
     using Mx = M<Real>;
@@ -588,7 +589,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             }
         }
 
-        line_volume<N, Real, VOL>(ph, hw, D, acc);
+        line_volume<N, Real, VOL, PIN>(ph, hw, D, acc);
         const Real scale = VOL == VOL_COLLOCATION ? -jac : (Real)-2 * jac;
 #pragma unroll
         for (int i = 0; i < N; i++)
@@ -743,30 +744,32 @@ __global__ __launch_bounds__(BLOCK, (Occ<P, Real, VOL, FULL>::waves)) void stage
     stage_tile<P, Real, FULL, VOL>(a, sc, tile_of_block(blockIdx.x, gridDim.x, a.xcd), blockIdx.x == 0);
 }
 
-// Persistent batched stage: many independent solves (own K, p, domain, dt)
-// in one launch.  tiles[] lists (instance, tile) pairs, heaviest first; each
-// workgroup walks the list with a grid stride.  args[3*i + stage-1] holds the
-// instance's stage arguments (its own device step state and wave-speed slots).
-// The batched kernel carries every degree's body; its register budget is the
-// largest one's (2 waves/SIMD in FP32, 1 in FP64).
+// Batched stage: many independent solves (own K, p, domain, dt) in one
+// launch.  tiles[] lists (instance, tile) pairs, heaviest first, one
+// workgroup per tile (a grid-stride loop over the list lets LLVM hoist the
+// loop-invariant work of all seven degree bodies out of the loop, which
+// costs ~70 registers and spills).  args[3*i + stage-1] holds instance i's
+// stage arguments (its own device step state and wave-speed slots).  The
+// kernel carries every degree's body; with the accumulator pin its register
+// budget fits 4 waves/SIMD in FP32 (2 in FP64).
 template <typename Real> struct BatchOcc { static constexpr int waves = sizeof(Real) == 4 ? 2 : 1; };
 
 template <typename Real, bool FULL, int VOL>
 __global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_kernel(
     const KArgs* __restrict__ args, const int2* __restrict__ tiles, int ntiles, int stage, StepCtl sc) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int2 tl = tiles[t];
-        const KArgs& a = args[(size_t)tl.x * 3 + (stage - 1)];
-        const bool lead = tl.y == 0;
-        switch (a.P) {
-            case 1: stage_tile<1, Real, FULL, VOL>(a, sc, tl.y, lead); break;
-            case 2: stage_tile<2, Real, FULL, VOL>(a, sc, tl.y, lead); break;
-            case 3: stage_tile<3, Real, FULL, VOL>(a, sc, tl.y, lead); break;
-            case 4: stage_tile<4, Real, FULL, VOL>(a, sc, tl.y, lead); break;
-            case 5: stage_tile<5, Real, FULL, VOL>(a, sc, tl.y, lead); break;
-            case 6: stage_tile<6, Real, FULL, VOL>(a, sc, tl.y, lead); break;
-            default: stage_tile<7, Real, FULL, VOL>(a, sc, tl.y, lead); break;
-        }
+    const int t = blockIdx.x;
+    if (t >= ntiles) return;
+    const int2 tl = tiles[t];
+    const KArgs& a = args[(size_t)tl.x * 3 + (stage - 1)];
+    const bool lead = tl.y == 0;
+    switch (a.P) {
+        case 1: stage_tile<1, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
+        case 2: stage_tile<2, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
+        case 3: stage_tile<3, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
+        case 4: stage_tile<4, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
+        case 5: stage_tile<5, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
+        case 6: stage_tile<6, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
+        default: stage_tile<7, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
     }
 }
 

@@ -28,14 +28,14 @@ DG_DECLARE_P(6)
 DG_DECLARE_P(7)
 #undef DG_DECLARE_P
 
-// Batched stage launchers (dgsem_batch.hip, one object per precision).
+// Batched stage launchers (dgsem_batch.hip, one object per precision): one
+// workgroup of `lanes` threads per tile.
 hipError_t launch_batch_stage_float(const KArgs* args, const int2* tiles, int ntiles, int stage, StepCtl sc,
-                                    int vol, int grid, size_t lds, hipStream_t s);
+                                    int vol, int lanes, size_t lds, hipStream_t s);
 hipError_t launch_batch_stage_double(const KArgs* args, const int2* tiles, int ntiles, int stage, StepCtl sc,
-                                     int vol, int grid, size_t lds, hipStream_t s);
-hipError_t batch_stage_occupancy_float(int vol, size_t lds, int* blocks_per_cu);
-hipError_t batch_stage_occupancy_double(int vol, size_t lds, int* blocks_per_cu);
+                                     int vol, int lanes, size_t lds, hipStream_t s);
 
 constexpr int PMAX_GPU = 7;
+constexpr int BATCH_BLOCK = 64;   // default lanes per workgroup of the batched stage kernel
 
 }  // namespace dg

@@ -429,6 +429,9 @@ static int grid_for(size_t n, int block) {
 }
 
 static int max_elems_per_block(int N) { return BLOCK / (2 * N); }
+// Batched launches: the most elements whose lines fit in a workgroup of
+// `lanes` threads (every degree's tile then fills about the same workgroup).
+static int batch_elems_per_block(int N, int lanes) { return std::max(1, lanes / (2 * N)); }
 
 // Default schedule: the largest tile that still gives >= 2048 workgroups
 // (8 per CU) and at least one full wave of line threads.
@@ -920,7 +923,7 @@ struct dg_batch {
     int2* d_tiles = nullptr;
     DevState* d_gather = nullptr;
     double* d_t0 = nullptr;
-    int ntiles = 0, grid = 0, grid_override = 0, occ_blocks = 1, cus = 1;
+    int ntiles = 0, lanes = BATCH_BLOCK, tiles_cap = 0;
     size_t lds = 0;
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
@@ -946,9 +949,9 @@ static int bfail(dg_batch* b, int code, const char* fmt, ...) {
 static hipError_t batch_launch_stage(dg_batch* b, int stage, StepCtl sc) {
     const bool fp32 = b->common.precision == DG_MIXED;
     return fp32 ? launch_batch_stage_float(b->d_args, b->d_tiles, b->ntiles, stage, sc, b->common.volume_flux,
-                                           b->grid, b->lds, b->cap)
+                                           b->lanes, b->lds, b->cap)
                 : launch_batch_stage_double(b->d_args, b->d_tiles, b->ntiles, stage, sc, b->common.volume_flux,
-                                            b->grid, b->lds, b->cap);
+                                            b->lanes, b->lds, b->cap);
 }
 
 static hipError_t batch_step(dg_batch* b, StepCtl sc) {
@@ -964,7 +967,8 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
         dg_ctx* c = b->ctx[i];
         const dg_config& f = c->cfg;
         const bool fp32 = f.precision == DG_MIXED;
-        LdsLayout L = lds_layout(c->N, max_elems_per_block(c->N), fp32 ? 4 : 8, true, false, false);
+        const int E = batch_elems_per_block(c->N, b->lanes);
+        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, true, false, false);
         for (int s = 1; s <= 3; s++) {
             KArgs a;
             memset(&a, 0, sizeof(a));
@@ -975,7 +979,7 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
             a.tab = fp32 ? c->d_tab32 : c->d_tab64;
             a.w64 = (const double*)c->d_tab64 + c->N2;
             a.st = c->d_st;
-            a.P = f.p; a.K = f.K; a.NEL = c->NEL; a.E = max_elems_per_block(c->N); a.xcd = 0;
+            a.P = f.p; a.K = f.K; a.NEL = c->NEL; a.E = E; a.xcd = 0;
             a.mode = MODE_STAGE; a.stage = s; a.vol = f.volume_flux; a.surf = f.surface_flux; a.bc = f.bc;
             a.av = 0; a.limiter = 0;
             a.h = c->h; a.gamma = f.gamma;
@@ -987,6 +991,42 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
             b->h_args[(size_t)i * 3 + (s - 1)] = a;
         }
     }
+}
+
+// (Re)build the tile list for the current lanes per workgroup: instance i
+// is cut into tiles of batch_elems_per_block(N_i, lanes) elements, heaviest
+// tiles first (cost of a tile ~ E (p+1)^3: pairs per line x lines); the LDS
+// size is the largest instance's.
+static hipError_t batch_tiles(dg_batch* b) {
+    std::vector<std::pair<double, int2>> tl;
+    size_t lds = 0;
+    for (int i = 0; i < b->n; i++) {
+        dg_ctx* c = b->ctx[i];
+        const int E = batch_elems_per_block(c->N, b->lanes);
+        const int nt = (c->NEL + E - 1) / E;
+        const double cost = (double)E * c->N * c->N * c->N;
+        for (int t = 0; t < nt; t++) tl.push_back({cost, make_int2(i, t)});
+        LdsLayout L = lds_layout(c->N, E, b->common.precision == DG_MIXED ? 4 : 8, true, false, false);
+        lds = std::max(lds, (size_t)L.total);
+    }
+    std::stable_sort(tl.begin(), tl.end(), [](const std::pair<double, int2>& x, const std::pair<double, int2>& y) {
+        return x.first > y.first;
+    });
+    std::vector<int2> tiles(tl.size());
+    for (size_t k = 0; k < tl.size(); k++) tiles[k] = tl[k].second;
+    if ((int)tiles.size() > b->tiles_cap) {
+        hipFree(b->d_tiles);
+        b->d_tiles = nullptr;
+        hipError_t e = hipMalloc(&b->d_tiles, tiles.size() * sizeof(int2));
+        if (e != hipSuccess) return e;
+        b->tiles_cap = (int)tiles.size();
+    }
+    b->ntiles = (int)tiles.size();
+    b->lds = lds;
+    hipError_t e = hipMemcpyAsync(b->d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice,
+                                  b->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(b->stream);   // the host vector goes out of scope
 }
 
 static int batch_status(dg_batch* b, int nsteps, double* t_inout, int* status_out) {
@@ -1035,47 +1075,20 @@ int dg_batch_create(const dg_config* cfgs, int n, void* stream, dg_batch** out) 
         int rc = dg_create(&cfgs[i], stream, &b->ctx[i]);
         if (rc) { dg_batch_destroy(b); return rc; }
     }
-    // tile list, heaviest first: cost of an element ~ (p+1)^3 (pairs per line x lines)
-    std::vector<std::pair<double, int2>> tl;
-    size_t lds = 0;
-    for (int i = 0; i < n; i++) {
-        dg_ctx* c = b->ctx[i];
-        const int E = max_elems_per_block(c->N);
-        const int nt = (c->NEL + E - 1) / E;
-        const double cost = (double)E * c->N * c->N * c->N;
-        for (int t = 0; t < nt; t++) tl.push_back({cost, make_int2(i, t)});
-        LdsLayout L = lds_layout(c->N, E, c0.precision == DG_MIXED ? 4 : 8, true, false, false);
-        lds = std::max(lds, (size_t)L.total);
-    }
-    std::stable_sort(tl.begin(), tl.end(), [](const std::pair<double, int2>& x, const std::pair<double, int2>& y) {
-        return x.first > y.first;
-    });
-    std::vector<int2> tiles(tl.size());
-    for (size_t k = 0; k < tl.size(); k++) tiles[k] = tl[k].second;
-    b->ntiles = (int)tiles.size();
-    b->lds = lds;
     b->h_args.resize((size_t)3 * n);
 #define BCK(x)                                                                 \
     do {                                                                       \
         if ((x) != hipSuccess) { dg_batch_destroy(b); return DG_ERR_HIP; }    \
     } while (0)
     BCK(hipMalloc(&b->d_args, (size_t)3 * n * sizeof(KArgs)));
-    BCK(hipMalloc(&b->d_tiles, tiles.size() * sizeof(int2)));
     BCK(hipMalloc(&b->d_gather, (size_t)n * sizeof(DevState)));
     BCK(hipMalloc(&b->d_t0, (size_t)n * sizeof(double)));
-    BCK(hipMemcpyAsync(b->d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream));
     BCK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
     BCK(hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming));
     BCK(hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming));
-    int dev = 0;
-    BCK(hipGetDevice(&dev));
-    BCK(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev));
-    BCK(c0.precision == DG_MIXED ? batch_stage_occupancy_float(c0.volume_flux, lds, &b->occ_blocks)
-                                 : batch_stage_occupancy_double(c0.volume_flux, lds, &b->occ_blocks));
+    BCK(batch_tiles(b));
     BCK(hipStreamSynchronize(b->stream));
 #undef BCK
-    b->occ_blocks = std::max(1, b->occ_blocks);
-    b->grid = std::min(b->ntiles, b->cus * b->occ_blocks);
     *out = b;
     return DG_OK;
 }
@@ -1107,12 +1120,17 @@ int dg_batch_init_state(dg_batch* b, const dg_ic_params* ics, double* const* Qs)
     return DG_OK;
 }
 
-int dg_batch_set_options(dg_batch* b, int use_graph, int grid_blocks) {
-    if (!b || grid_blocks < 0) return DG_ERR_INVALID_ARG;
+int dg_batch_set_options(dg_batch* b, int use_graph, int lanes) {
+    if (!b) return DG_ERR_INVALID_ARG;
+    if (lanes == 0) lanes = BATCH_BLOCK;
+    if (lanes != 64 && lanes != 128 && lanes != 256)
+        return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_set_options: lanes %d not 64, 128 or 256", lanes);
     b->use_graph = use_graph != 0;
-    b->grid_override = grid_blocks;
-    b->grid = grid_blocks > 0 ? std::min(grid_blocks, b->ntiles) : std::min(b->ntiles, b->cus * b->occ_blocks);
-    if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }   // grid is baked into the graph
+    if (lanes != b->lanes) {
+        b->lanes = lanes;
+        BHIPCK(b, batch_tiles(b));
+    }
+    if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }   // tiles are baked into the graph
     return DG_OK;
 }
 
@@ -1127,8 +1145,8 @@ int dg_batch_advance(dg_batch* b, double* const* Qs, int nsteps, const double* c
     if (t_inout) BHIPCK(b, hipMemcpyAsync(b->d_t0, t_inout, b->n * sizeof(double), hipMemcpyHostToDevice, b->stream));
     hipLaunchKernelGGL(batch_reset_kernel, dim3((b->n + 63) / 64), dim3(64), 0, b->stream, b->d_args, b->n,
                        t_inout ? b->d_t0 : nullptr);
-    hipLaunchKernelGGL(batch_lambda_kernel, dim3(b->grid), dim3(BLOCK), 0, b->stream, b->d_args, b->d_tiles,
-                       b->ntiles);
+    hipLaunchKernelGGL(batch_lambda_kernel, dim3(std::min(b->ntiles, 8192)), dim3(BLOCK), 0, b->stream, b->d_args,
+                       b->d_tiles, b->ntiles);
     BHIPCK(b, hipGetLastError());
     // stage launches run on the capture stream, ordered after the work above
     BHIPCK(b, hipEventRecord(b->ev_in, b->stream));
@@ -1169,19 +1187,17 @@ int dg_batch_advance_each(dg_batch* b, double* const* Qs, int nsteps, const doub
     return first;
 }
 
-int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* grid_blocks) {
+int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* lanes) {
     if (!b || !Qs || !cfl) return DG_ERR_INVALID_ARG;
-    const int base = b->cus * b->occ_blocks;
-    const int cands[] = {base, base / 2, 2 * base, 4 * base};
+    const int cands[] = {64, 128, 256};
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     float best = 1e30f;
-    int best_grid = b->grid;
+    int best_lanes = b->lanes;
     int rc = DG_OK;
-    for (int g : cands) {
-        if (g < 1) continue;
-        dg_batch_set_options(b, b->use_graph, g);
+    for (int l : cands) {
+        if ((rc = dg_batch_set_options(b, b->use_graph, l))) break;
         // note: the states are advanced by 4 steps per candidate at cfl[i]
         if ((rc = dg_batch_advance(b, Qs, 1, cfl, 0.0, nullptr, nullptr))) break;   // warm-up + graph
         hipEventRecord(e0, b->stream);
@@ -1190,13 +1206,13 @@ int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* gr
         hipEventSynchronize(e1);
         float ms = 0.f;
         hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best) { best = ms; best_grid = g; }
+        if (ms < best) { best = ms; best_lanes = l; }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    dg_batch_set_options(b, b->use_graph, best_grid);
-    if (grid_blocks) *grid_blocks = b->grid;
-    return rc;
+    int rc2 = dg_batch_set_options(b, b->use_graph, best_lanes);
+    if (lanes) *lanes = b->lanes;
+    return rc ? rc : rc2;
 }
 
 }  // extern "C"

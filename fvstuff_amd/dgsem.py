@@ -261,7 +261,7 @@ class Solver:
 
 class Batch:
     """Many independent solves (own K, p, domain, IC, CFL) stepped together:
-    one persistent launch per stage for all instances, graph-replayed."""
+    one launch per stage for all instances (one workgroup per tile), graph-replayed."""
 
     def __init__(self, cfgs, device="cuda:0", stream=None):
         if not torch.cuda.is_available():
@@ -327,8 +327,8 @@ class Batch:
         """The same steps, one dg_advance per instance (for comparison)."""
         return self._advance(self.lib.dg_batch_advance_each, Qs, nsteps, cfl, t_end, t)
 
-    def set_options(self, use_graph=True, grid_blocks=0):
-        self._check(self.lib.dg_batch_set_options(self._b, int(bool(use_graph)), int(grid_blocks)))
+    def set_options(self, use_graph=True, lanes=0):
+        self._check(self.lib.dg_batch_set_options(self._b, int(bool(use_graph)), int(lanes)))
 
     def autotune(self, Qs, cfl):
         cfl = [float(cfl)] * self.n if isinstance(cfl, (int, float)) else [float(c) for c in cfl]

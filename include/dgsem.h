@@ -208,11 +208,12 @@ int         dg_autotune(dg_ctx* ctx, int* elems_per_block, int* xcd_remap);
 
 /* ── Batched independent solves (no counterpart in the reference, which runs
  * one solve per process): ensembles of instances that differ in K, p, domain,
- * IC and CFL but share the flux / boundary / precision switches.  Every stage
- * of every instance is one persistent launch over a tile list (heaviest tiles
- * first); each instance keeps its own device-side dt and time.  The stage
- * launches of the time loop are replayed from a captured HIP graph.  AV and
- * the limiter are not available in batched mode (DG_ERR_INVALID_ARG). */
+ * IC and CFL but share the flux / boundary / precision switches.  Each stage
+ * of all instances is one launch over a tile list (heaviest tiles first, one
+ * workgroup per tile, every degree's code in the same kernel); each instance
+ * keeps its own device-side dt and time.  The stage launches of the time loop
+ * are replayed from a captured HIP graph.  AV and the limiter are not
+ * available in batched mode (DG_ERR_INVALID_ARG). */
 typedef struct dg_batch dg_batch;
 
 int         dg_batch_create(const dg_config* cfgs, int n, void* stream, dg_batch** out);
@@ -230,11 +231,12 @@ int         dg_batch_advance(dg_batch* b, double* const* Qs, int nsteps, const d
 /* The same steps issued instance by instance with dg_advance (for comparison). */
 int         dg_batch_advance_each(dg_batch* b, double* const* Qs, int nsteps, const double* cfl, double t_end,
                                   double* t_inout, int* status_out);
-/* Tuning: use_graph (0/1), persistent grid size in workgroups (0 = occupancy x CUs). */
-int         dg_batch_set_options(dg_batch* b, int use_graph, int grid_blocks);
-/* Time a few persistent grid sizes on the given states (they are advanced by
- * a few steps) and keep the fastest; writes it to *grid_blocks if non-NULL. */
-int         dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* grid_blocks);
+/* Tuning: use_graph (0/1); lanes per workgroup, 64 / 128 / 256 (0 = 64):
+ * instance i's tiles hold max(1, lanes / (2(p_i+1))) elements. */
+int         dg_batch_set_options(dg_batch* b, int use_graph, int lanes);
+/* Time the three workgroup sizes on the given states (they are advanced by
+ * a few steps) and keep the fastest; writes it to *lanes if non-NULL. */
+int         dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* lanes);
 
 #ifdef __cplusplus
 }

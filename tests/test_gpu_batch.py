@@ -1,4 +1,4 @@
-"""Batched persistent launch: every instance must match its own single-solve
+"""Batched launch: every instance must match its own single-solve
 run bit for bit (same per-tile arithmetic), with or without graph replay,
 and per-instance dt / end time / status must be honoured."""
 import numpy as np
@@ -53,6 +53,27 @@ def test_batch_graph_and_each_agree(D):
         assert ts == runs[0][0]
         for a, c in zip(Q, runs[0][1]):
             assert np.array_equal(a, c)
+
+
+def test_batch_lanes_agree(D):
+    """Tiles of 64, 128 or 256 lanes cut the instances differently; the
+    results are the same bit for bit."""
+    cfgs, ics, cfls = ensemble(n=8, seed=3, k_range=(5, 17), degrees=(1, 3, 4, 5, 7))
+    b = D.Batch(cfgs)
+    ref = None
+    for lanes in (64, 128, 256):
+        b.set_options(use_graph=True, lanes=lanes)
+        Qs = b.init_states(ics)
+        ts = b.advance(Qs, 6, cfls)
+        got = (ts, host(Qs))
+        if ref is None:
+            ref = got
+            continue
+        assert got[0] == ref[0]
+        for a, c in zip(got[1], ref[1]):
+            assert np.array_equal(a, c)
+    with pytest.raises(D.DGError):
+        b.set_options(lanes=96)
 
 
 def test_batch_end_time_and_status(D):
