@@ -47,7 +47,8 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--schedule", default="",
                     help="E,X: elements per workgroup, XCD remap (implies --no-autotune)")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=15,
+                    help="SSP-RK3 steps of the CPU oracle baseline (about 10 s on 16 host threads)")
     ap.add_argument("--workload", default="vortex", choices=["vortex", "kh", "batched", "copy"],
                     help="vortex: the headline line; kh: Kelvin-Helmholtz (--K, --p, --t-end); "
                          "batched: seeded ensemble of vortex instances; copy: device copy probe")

@@ -165,18 +165,6 @@ template <typename Real> struct Phys {
         if (ub >= (Real)Mx::series_thr) inv_b_ln = Mx::div(Mx::ln(Mx::div(bL, bR)), bL - bR);
     }
 
-    // Ranocha EC + KEP two-point flux along x (y-lines pass u,v swapped).
-    __device__ __forceinline__ void ranocha(const Prim<Real>& L, const Prim<Real>& R, Real F[4]) const {
-        Real r_ln, inv_z_ln;
-        pair_means(L.r, R.r, L.z, R.z, r_ln, inv_z_ln);
-        Real u_av = (Real)0.5 * (L.u + R.u), v_av = (Real)0.5 * (L.v + R.v);
-        Real p_av = (Real)0.5 * (L.p + R.p);
-        Real eint = (Real)0.5 * (L.u * R.u + L.v * R.v) + inv_gm1 * inv_z_ln;
-        Real f1 = r_ln * u_av;
-        F[0] = f1; F[1] = f1 * u_av + p_av; F[2] = f1 * v_av;
-        F[3] = f1 * eint + (Real)0.5 * (L.p * R.u + R.p * L.u);
-    }
-
     // Ranocha flux on half-scaled node states h = (rho, u, v, p, rho/p)/2, the
     // form the line loop uses: (a+b)/2 = aL/2 + aR/2 needs no multiply, one
     // reciprocal of the product (rho_L+rho_R)(z_L+z_R) serves both log means,
