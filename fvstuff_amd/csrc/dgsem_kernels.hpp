@@ -538,13 +538,14 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         for (int q = 0; q < 4; q++) acc[i][q] = 0;
 
     if (is_line) {
+        const int yd = ydir ? 1 : 0;
         Prim<Real> hw[N];    // half-scaled line states
         const Real* eb = sW + (size_t)le * N2 * 5;
 #pragma unroll
         for (int k = 0; k < N; k++) {
+            // y-lines read u and v swapped (the rotation is in the LDS index)
             const Real* p = eb + (lbase + k * lstride) * 5;
-            hw[k].r = p[0]; hw[k].u = p[1]; hw[k].v = p[2]; hw[k].p = p[3]; hw[k].z = p[4];
-            rot(hw[k], ydir);
+            hw[k].r = p[0]; hw[k].u = p[1 + yd]; hw[k].v = p[2 - yd]; hw[k].p = p[3]; hw[k].z = p[4];
         }
 
         // ── surface terms at both ends of the line (one call site for the
@@ -564,7 +565,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                         // conversion, so the face sees bit-identical states
                         const int elnb = f ? hel[1] : hel[0], ndnb = f ? hnd[1] : hnd[0];
                         const Real* pn = sW + ((size_t)(elnb - el0) * N2 + ndnb) * 5;
-                        nb.r = pn[0]; nb.u = pn[1]; nb.v = pn[2]; nb.p = pn[3]; nb.z = pn[4];
+                        nb.r = pn[0]; nb.u = pn[1 + yd]; nb.v = pn[2 - yd]; nb.p = pn[3]; nb.z = pn[4];
                         nb = doubled(nb);
                     } else {
                         double U[4], W[4];
@@ -572,8 +573,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                         for (int q = 0; q < 4; q++) U[q] = f ? Hn[1][q] : Hn[0][q];
                         Cons64::to_prim(U, W, g);
                         nb = make_prim<Real>(W);
+                        rot(nb, ydir);
                     }
-                    rot(nb, ydir);
                 }
                 const Prim<Real> L = f ? own : nb, R = f ? nb : own;
                 Real Fs[4], fo[4];
@@ -633,13 +634,6 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                     acc[i][q] -= jac * winv[i] * s;
                 }
         }
-        // rotate back: swap momentum components of y-lines
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-            Real m1 = acc[i][1], m2 = acc[i][2];
-            acc[i][1] = ydir ? m2 : m1;
-            acc[i][2] = ydir ? m1 : m2;
-        }
     }
     // U^n for the update (stages 2, 3), issued now so that the loads are in
     // flight across the barriers and the LDS exchange below
@@ -665,11 +659,13 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     }
     __syncthreads();
     if (is_line && ydir) {
+        // y-line momentum components are in the rotated frame: (v, u)
         Real* r = sR + (size_t)le * 4 * N2;
+        constexpr int qs[4] = {0, 2, 1, 3};
 #pragma unroll
         for (int i = 0; i < N; i++)
 #pragma unroll
-            for (int q = 0; q < 4; q++) r[q * N2 + lbase + i * lstride] += acc[i][q];
+            for (int q = 0; q < 4; q++) r[qs[q] * N2 + lbase + i * lstride] += acc[i][q];
     }
     __syncthreads();
 
