@@ -66,8 +66,70 @@ struct KArgs {
     int mode, stage, vol, surf, bc, av, limiter, slot_cur, slot_next;
     int check_blowup;    // stage 1: the wave speed in slot_cur came from a step of this call
     double h, gamma, ca, cb, cfl, t_end, av_cav, av_k, eps_rho, eps_p;
+    // derived constants, computed once on the host (set_derived): 2/h,
+    // g-1, 1/(g-1), 1/K, the energy floor 1e-14/(g-1); FP32 copies for the
+    // mixed form
+    double jac, gm1, inv_gm1, inv_K, efloor;
+    float jac_f, g_f, gm1_f, inv_gm1_f;
     int off_U, off_av, off_lam, off_el;   // byte offsets of the LDS regions
 };
+
+// Host side: fill the derived constants of an argument block whose h and
+// gamma are set (the same IEEE divisions the device would do).
+inline void set_derived(KArgs& a) {
+    a.jac = 2.0 / a.h;
+    a.gm1 = a.gamma - 1.0;
+    a.inv_gm1 = 1.0 / (a.gamma - 1.0);
+    a.inv_K = 1.0 / (double)a.K;
+    a.efloor = 1e-14 / (a.gamma - 1.0);
+    a.jac_f = (float)a.jac;
+    a.g_f = (float)a.gamma;
+    a.gm1_f = (float)a.gm1;
+    a.inv_gm1_f = (float)a.inv_gm1;
+}
+
+// a derived constant in the compute precision (no conversion for FP32)
+template <typename Real>
+__device__ __forceinline__ Real kconst(double d, float f) {
+    if (sizeof(Real) == 4) return (Real)f;
+    return (Real)d;
+}
+
+// Global FP64 stream access at a 32-bit byte offset from a uniform base held
+// in scalar registers (global_load/store with a scalar base and a 32-bit
+// vector offset); the callers keep the offset within one tile of one
+// variable.  `sbase` materialises the base in SGPRs through an empty asm (no
+// instruction), so that the address arithmetic is not re-associated into
+// 64-bit vector adds.
+#define DG_GLOBAL __attribute__((address_space(1)))
+typedef DG_GLOBAL const double* gcptr;
+typedef DG_GLOBAL double* gptr;
+__device__ __forceinline__ gcptr sbase(const double* p) {
+    gcptr g = (gcptr)p;
+    asm("" : "+s"(g));
+    return g;
+}
+__device__ __forceinline__ gptr sbase(double* p) {
+    gptr g = (gptr)p;
+    asm("" : "+s"(g));
+    return g;
+}
+__device__ __forceinline__ double ld_at(gcptr base, unsigned byte_off) {
+    return *(gcptr)((DG_GLOBAL const char*)base + byte_off);
+}
+__device__ __forceinline__ void st_at(gptr base, unsigned byte_off, double v) {
+    *(gptr)((DG_GLOBAL char*)base + byte_off) = v;
+}
+
+// (row, column) of element el of a K x K grid without an integer division:
+// the FP64 quotient is within one of the true one for el < 2^31.
+__device__ __forceinline__ void elem_rc(int el, int K, double inv_K, int& ej, int& ei) {
+    int q = (int)((double)el * inv_K);
+    int r = el - q * K;
+    if (r < 0) { q -= 1; r += K; }
+    if (r >= K) { q += 1; r -= K; }
+    ej = q; ei = r;
+}
 
 // Per-launch step control of the fused stepping (ping-pong wave-speed slots).
 struct StepCtl {
@@ -222,51 +284,41 @@ __device__ __forceinline__ void pin_pair(Real (&acc)[N][4], int i, int k) {
 }
 
 // ── symmetric pair loop of one line: F#(U_i,U_k) once per i < k ──────────
+// Dv is the volume matrix (D scaled by the volume term's factor).  acc[0] and
+// acc[N-1] hold their first terms already; node k of 0 < k < N-1 receives its
+// first term from the pair (0, k), which initialises it.
 template <int N, typename Real, int VOL, bool PIN>
-__device__ __forceinline__ void line_pairs(const Phys<Real>& ph, const Prim<Real> (&hw)[N], const Real* D,
+__device__ __forceinline__ void line_pairs(const Phys<Real>& ph, const Prim<Real> (&hw)[N], const Real* Dv,
                                            Real (&acc)[N][4]) {
+    Real c[4] = {0, 0, 0, 0};
     if (VOL == VOL_RANOCHA) {
-        const Real c[4] = {ph.inv_gm1, ph.inv_gm1 * (Real)(1.0 / 3.0), ph.inv_gm1 * (Real)(1.0 / 5.0),
-                           ph.inv_gm1 * (Real)(1.0 / 7.0)};
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-#pragma unroll
-            for (int k = i + 1; k < N; k++) {
-                Real F[4];
-                Phys<Real>::ranocha_half(hw[i], hw[k], c, ph.inv_gm1, F);
-                const Real dik = D[i * N + k], dki = D[k * N + i];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    acc[i][q] += dik * F[q];
-                    acc[k][q] += dki * F[q];
-                }
-                pin_pair<N, PIN>(acc, i, k);
-            }
-        }
-        return;
+        c[0] = ph.inv_gm1; c[1] = ph.inv_gm1 * (Real)(1.0 / 3.0);
+        c[2] = ph.inv_gm1 * (Real)(1.0 / 5.0); c[3] = ph.inv_gm1 * (Real)(1.0 / 7.0);
     }
 #pragma unroll
     for (int i = 0; i < N; i++) {
 #pragma unroll
         for (int k = i + 1; k < N; k++) {
             Real F[4];
-            ph.chandrashekar(doubled(hw[i]), doubled(hw[k]), F);
-            const Real dik = D[i * N + k], dki = D[k * N + i];
+            if (VOL == VOL_RANOCHA) Phys<Real>::ranocha_half(hw[i], hw[k], c, ph.inv_gm1, F);
+            else ph.chandrashekar(doubled(hw[i]), doubled(hw[k]), F);
+            const Real dik = Dv[i * N + k], dki = Dv[k * N + i];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 acc[i][q] += dik * F[q];
-                acc[k][q] += dki * F[q];
+                if (i == 0 && k < N - 1) acc[k][q] = dki * F[q];
+                else acc[k][q] += dki * F[q];
             }
             pin_pair<N, PIN>(acc, i, k);
         }
     }
 }
 
-// ── volume contribution of one line: acc[i] = sum_k D_ik F#(U_i,U_k) ──────
+// ── volume contribution of one line plus its two face terms:
+//    acc[i] = sum_k Dv_ik F#(U_i,U_k) + [i = 0] fm + [i = N-1] fp ──────────
 template <int N, typename Real, int VOL, bool PIN>
-__device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Real> (&hw)[N], const Real* D,
-                                            Real (&acc)[N][4]) {
-    using Mx = M<Real>;
+__device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Real> (&hw)[N], const Real* Dv,
+                                            const Real (&fm)[4], const Real (&fp)[4], Real (&acc)[N][4]) {
     if (VOL == VOL_COLLOCATION) {
         // linear contraction sum_k D_ik f(U_k) (:431-439)
 #pragma unroll
@@ -275,25 +327,28 @@ __device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Rea
             ph.template flux<0>(doubled(hw[k]), f);
 #pragma unroll
             for (int i = 0; i < N; i++) {
-                const Real dik = D[i * N + k];
+                const Real dik = Dv[i * N + k];
 #pragma unroll
-                for (int q = 0; q < 4; q++) acc[i][q] += dik * f[q];
+                for (int q = 0; q < 4; q++) {
+                    if (k > 0) acc[i][q] += dik * f[q];
+                    else if (i == 0) acc[i][q] = dik * f[q] + fm[q];
+                    else if (i == N - 1) acc[i][q] = dik * f[q] + fp[q];
+                    else acc[i][q] = dik * f[q];
+                }
             }
         }
         return;
     }
     // diagonal: F#(U_i,U_i) = f(U_i) (consistency).  On LGL nodes D_ii = 0
-    // except D_00 = -p(p+1)/4 and D_pp = p(p+1)/4 (the negative-sum
-    // construction leaves ~1e-16 on the interior diagonal, dropped here).
+    // except D_00 = -p(p+1)/4 = -1/(2 w_0) and D_pp = 1/(2 w_p), so the
+    // split form's 2 D_00 f(U_0) equals the -f(U_0)/w_0 of the strong-form
+    // surface term with the opposite sign: the two cancel exactly, and the
+    // face terms fm, fp carry the Riemann flux alone (weak form at the end
+    // nodes; the interior diagonal, ~1e-16 from the negative-sum
+    // construction, is dropped).
 #pragma unroll
-    for (int i = 0; i < N; i += N - 1) {
-        Real f[4];
-        ph.template flux<0>(doubled(hw[i]), f);
-        const Real dii = D[i * N + i];
-#pragma unroll
-        for (int q = 0; q < 4; q++) acc[i][q] += dii * f[q];
-    }
-    line_pairs<N, Real, VOL, PIN>(ph, hw, D, acc);
+    for (int q = 0; q < 4; q++) { acc[0][q] = fm[q]; acc[N - 1][q] = fp[q]; }
+    line_pairs<N, Real, VOL, PIN>(ph, hw, Dv, acc);
 }
 
 // Register budget: a line keeps its p+1 node states and 4(p+1) accumulators
@@ -324,19 +379,23 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     const int nE = min(E, a.NEL - el0);
     const size_t S = (size_t)a.NEL * N2;
     const double g = a.gamma;
+    const Gas64 gs{g, a.gm1, a.efloor};
 
     const Real* D = (const Real*)a.tab;
     const Real* winv = D + N * N + N;
+    const Real* Dv = D + N * N + 2 * N;             // volume matrix
+    // the tile's first node in every FP64 stream (variable q at + q*S)
+    const size_t tile_off = (size_t)el0 * N2;
 
-    const Phys<Real> ph{(Real)g, (Real)(g - 1.0), (Real)(1.0 / (g - 1.0))};
+    const Phys<Real> ph{kconst<Real>(g, a.g_f), kconst<Real>(a.gm1, a.gm1_f), kconst<Real>(a.inv_gm1, a.inv_gm1_f)};
 
     // LDS regions
     Real* sW = (Real*)smem;                          // [E][N2][5] prims; reused as sR [E][4][N2]
     Real* sR = sW;
     double* sU = (double*)(smem + a.off_U);         // [E][4][N2] FP64 stage input, then stage values
     Real* sDu = (Real*)(smem + a.off_av);           // [E][N2] AV: du/dx
-    Real* sDv = sDu + (size_t)E * N2;               // [E][N2] AV: dv/dy
-    Real* sNu = sDv + (size_t)E * N2;               // [E]     AV: element viscosity
+    Real* sDv = sDu + E * N2;                       // [E][N2] AV: dv/dy
+    Real* sNu = sDv + E * N2;                       // [E]     AV: element viscosity
     unsigned long long* sLamE = (unsigned long long*)(smem + a.off_lam);   // [E] max wave speed (bits)
     unsigned long long* sThE = sLamE + E;           // [E] max sensor (bits of the double)
     double* sEl = (double*)(smem + a.off_el);       // per-element scratch
@@ -358,7 +417,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     const int lbase = ydir ? lidx : lidx * N;
     const int lstride = ydir ? N : 1;
     const bool line_live = is_line && le < nE;
-    const Real jac = (Real)(2.0 / a.h);
+    const Real jac = kconst<Real>(a.jac, a.jac_f);
 
     // Face neighbours of the line's two end nodes: inside the tile they are
     // read from LDS in phase B; outside it their FP64 state is loaded here,
@@ -367,8 +426,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     int hk[2] = {0, 0}, hnd[2] = {0, 0}, hel[2] = {0, 0};
     double Hn[2][4];
     if (line_live) {
-        const int el = el0 + le;
-        const int ej = el / a.K, ei = el - ej * a.K;
+        int ej, ei;
+        elem_rc(el0 + le, a.K, a.inv_K, ej, ei);
         const int cpos = ydir ? ej : ei;
 #pragma unroll
         for (int f = 0; f < 2; f++) {
@@ -394,13 +453,15 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     constexpr int JMAX = (N + 1) / 2;
     const bool staged = a.mode == MODE_STAGE;
     double Ul[JMAX][4];
+    gcptr upq[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) upq[q] = sbase(a.Up + tile_off + q * S);
 #pragma unroll
     for (int j = 0; j < JMAX; j++) {
         const int s = tid + j * nthr;
         if (s < nE * N2) {
-            const size_t o = (size_t)el0 * N2 + s;
 #pragma unroll
-            for (int q = 0; q < 4; q++) Ul[j][q] = a.Up[q * S + o];
+            for (int q = 0; q < 4; q++) Ul[j][q] = ld_at(upq[q], (unsigned)s * 8u);
         }
     }
     // ── fused dt (device side, :1026-1035) ──
@@ -481,12 +542,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         if (s < nE * N2) {
             if (staged) {
 #pragma unroll
-                for (int q = 0; q < 4; q++) sU[(size_t)e * 4 * N2 + q * N2 + nd] = Ul[j][q];
+                for (int q = 0; q < 4; q++) sU[e * 4 * N2 + q * N2 + nd] = Ul[j][q];
             }
-            Cons64::to_prim(Ul[j], W, g);
+            Cons64::to_prim(Ul[j], W, gs);
         }
         Prim<Real> w = make_prim<Real>(W);
-        Real* p = sW + (size_t)s * 5;
+        Real* p = sW + s * 5;
         p[0] = (Real)0.5 * w.r; p[1] = (Real)0.5 * w.u; p[2] = (Real)0.5 * w.v;
         p[3] = (Real)0.5 * w.p; p[4] = (Real)0.5 * w.z;
         if (FULL && need_lam && s < nE * N2) atomicMax(&sLamE[e], dbits(Cons64::wave_speed(W, g)));
@@ -499,7 +560,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             Real un[N];
 #pragma unroll
             for (int k = 0; k < N; k++)
-                un[k] = (Real)2 * sW[((size_t)le * N2 + lbase + k * lstride) * 5 + (ydir ? 2 : 1)];
+                un[k] = (Real)2 * sW[(le * N2 + lbase + k * lstride) * 5 + (ydir ? 2 : 1)];
             Real* dst = ydir ? sDv : sDu;
 #pragma unroll
             for (int i = 0; i < N; i++) {
@@ -518,7 +579,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             const int s = tid + j * nthr;
             if (s >= nE * N2) continue;
             const int e = s / N2;
-            const Real* p = sW + (size_t)s * 5;
+            const Real* p = sW + s * 5;
             Real rho = Mx::fmax_((Real)2 * p[0], (Real)1e-12), c2 = ph.g * ((Real)2 * p[3]) / rho;
             Real divu = Mx::fmin_(jac * (sDu[s] + sDv[s]), (Real)0), du2 = divu * divu;
             Real th = du2 / (du2 + (Real)a.av_k * c2 * dx2inv + (Real)1e-30);
@@ -531,16 +592,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     }
 
     // ── B: line phase ──
-    Real acc[N][4];
-#pragma unroll
-    for (int i = 0; i < N; i++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) acc[i][q] = 0;
+    Real acc[N][4];     // set by line_volume (lines only; other lanes write nothing)
 
     if (is_line) {
         const int yd = ydir ? 1 : 0;
         Prim<Real> hw[N];    // half-scaled line states
-        const Real* eb = sW + (size_t)le * N2 * 5;
+        const Real* eb = sW + le * N2 * 5;
 #pragma unroll
         for (int k = 0; k < N; k++) {
             // y-lines read u and v swapped (the rotation is in the LDS index)
@@ -564,42 +621,45 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                         // neighbour staged by this tile: the same FP64 -> prim
                         // conversion, so the face sees bit-identical states
                         const int elnb = f ? hel[1] : hel[0], ndnb = f ? hnd[1] : hnd[0];
-                        const Real* pn = sW + ((size_t)(elnb - el0) * N2 + ndnb) * 5;
+                        const Real* pn = sW + ((elnb - el0) * N2 + ndnb) * 5;
                         nb.r = pn[0]; nb.u = pn[1 + yd]; nb.v = pn[2 - yd]; nb.p = pn[3]; nb.z = pn[4];
                         nb = doubled(nb);
                     } else {
                         double U[4], W[4];
 #pragma unroll
                         for (int q = 0; q < 4; q++) U[q] = f ? Hn[1][q] : Hn[0][q];
-                        Cons64::to_prim(U, W, g);
+                        Cons64::to_prim(U, W, gs);
                         nb = make_prim<Real>(W);
                         rot(nb, ydir);
                     }
                 }
-                const Prim<Real> L = f ? own : nb, R = f ? nb : own;
-                Real Fs[4], fo[4];
-                if (a.surf == SURF_RUSANOV) ph.rusanov(0, L, R, Fs);
-                else ph.hllc(0, L, R, Fs);
-                ph.template flux<0>(own, fo);
+                // split forms: F*(face) (the diagonal volume term cancels the
+                // -f(own) of the strong-form surface term exactly on LGL
+                // nodes); collocation: F*(face) - f(own)
+                constexpr bool JUMP = VOL == VOL_COLLOCATION;
+                Real dj[4];
+                if (a.surf == SURF_RUSANOV) {
+                    ph.template rusanov_side<JUMP>(own, nb, f != 0, dj);
+                } else {
+                    const Prim<Real> L = f ? own : nb, R = f ? nb : own;
+                    ph.hllc(0, L, R, dj);
+                    if (JUMP) {
+                        Real fo[4];
+                        ph.template flux<0>(own, fo);
+#pragma unroll
+                        for (int q = 0; q < 4; q++) dj[q] -= fo[q];
+                    }
+                }
                 const Real c = (f ? -jac : jac) * winv[f ? N - 1 : 0];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    Real v = c * (Fs[q] - fo[q]);
+                    Real v = c * dj[q];
                     if (f) cp[q] = v; else cm[q] = v;
                 }
             }
         }
 
-        line_volume<N, Real, VOL, PIN>(ph, hw, D, acc);
-        const Real scale = VOL == VOL_COLLOCATION ? -jac : (Real)-2 * jac;
-#pragma unroll
-        for (int i = 0; i < N; i++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) acc[i][q] *= scale;
-        if (line_live) {
-#pragma unroll
-            for (int q = 0; q < 4; q++) { acc[0][q] += cm[q]; acc[N - 1][q] += cp[q]; }
-        }
+        line_volume<N, Real, VOL, PIN>(ph, hw, Dv, cm, cp, acc);
 
         // ── artificial viscosity along the line: -jac W^-1 D^T W (nu jac D Uc) ──
         if (FULL && a.av) {
@@ -639,19 +699,21 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     // flight across the barriers and the LDS exchange below
     double U0l[JMAX][4];
     if (staged && a.ca != 0.0) {
+        gcptr u0q[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) u0q[q] = sbase(a.U0 + tile_off + q * S);
 #pragma unroll
         for (int j = 0; j < JMAX; j++) {
             const int s = tid + j * nthr;
             if (s < nE * N2) {
-                const size_t o = (size_t)el0 * N2 + s;
 #pragma unroll
-                for (int q = 0; q < 4; q++) U0l[j][q] = a.U0[q * S + o];
+                for (int q = 0; q < 4; q++) U0l[j][q] = ld_at(u0q[q], (unsigned)s * 8u);
             }
         }
     }
     __syncthreads();   // every read of sW is done: reuse it as sR
     if (is_line && !ydir) {
-        Real* r = sR + (size_t)le * 4 * N2;
+        Real* r = sR + le * 4 * N2;
 #pragma unroll
         for (int i = 0; i < N; i++)
 #pragma unroll
@@ -660,7 +722,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     __syncthreads();
     if (is_line && ydir) {
         // y-line momentum components are in the rotated frame: (v, u)
-        Real* r = sR + (size_t)le * 4 * N2;
+        Real* r = sR + le * 4 * N2;
         constexpr int qs[4] = {0, 2, 1, 3};
 #pragma unroll
         for (int i = 0; i < N; i++)
@@ -674,7 +736,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         for (int s = tid; s < nE * N2; s += nthr) {
             const int e = s / N2, nd = s - e * N2;
             const size_t o = (size_t)(el0 + e) * N2 + nd;
-            const Real* r = sR + (size_t)e * 4 * N2 + nd;
+            const Real* r = sR + e * 4 * N2 + nd;
 #pragma unroll
             for (int q = 0; q < 4; q++) a.out[q * S + o] = (double)r[q * N2];
         }
@@ -686,14 +748,16 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     const bool lim = FULL && a.limiter;
     const bool want_lambda = a.stage == 3;
     double lmax = 0.0;
+    gptr outq[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) outq[q] = sbase(a.out + tile_off + q * S);
 #pragma unroll
     for (int j = 0; j < JMAX; j++) {
         const int s = tid + j * nthr;
         if (s >= nE * N2) continue;
         const int e = s / N2, nd = s - e * N2;
-        const size_t o = (size_t)el0 * N2 + s;
-        const Real* r = sR + (size_t)e * 4 * N2 + nd;
-        double* up = sU + (size_t)e * 4 * N2 + nd;
+        const Real* r = sR + e * 4 * N2 + nd;
+        double* up = sU + e * 4 * N2 + nd;
         double Us[4];
         if (a.ca != 0.0) {
 #pragma unroll
@@ -702,16 +766,16 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
 #pragma unroll
             for (int q = 0; q < 4; q++) Us[q] = up[q * N2] + dt * (double)r[q * N2];
         }
-        Cons64::sanitize_cons(Us, g);
+        Cons64::sanitize_cons(Us, gs);
         if (lim) {
 #pragma unroll
             for (int q = 0; q < 4; q++) up[q * N2] = Us[q];
         } else {
 #pragma unroll
-            for (int q = 0; q < 4; q++) a.out[q * S + o] = Us[q];
+            for (int q = 0; q < 4; q++) st_at(outq[q], (unsigned)s * 8u, Us[q]);
             if (want_lambda) {
                 double W[4], ri;
-                Cons64::to_prim(Us, W, g, ri);
+                Cons64::to_prim(Us, W, gs, ri);
                 lmax = nanmax(lmax, Cons64::wave_speed(W, g, ri));
             }
         }
@@ -720,13 +784,15 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         limit_block<N>(sU, a.w64, nE, g, a.eps_rho, a.eps_p, sBar, sTrho, sTheta);
         for (int s = tid; s < nE * N2; s += nthr) {
             const int e = s / N2, nd = s - e * N2;
-            const size_t o = (size_t)el0 * N2 + s;
             double Us[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) { Us[q] = sU[(size_t)e * 4 * N2 + q * N2 + nd]; a.out[q * S + o] = Us[q]; }
+            for (int q = 0; q < 4; q++) {
+                Us[q] = sU[e * 4 * N2 + q * N2 + nd];
+                st_at(outq[q], (unsigned)s * 8u, Us[q]);
+            }
             if (want_lambda) {
                 double W[4], ri;
-                Cons64::to_prim(Us, W, g, ri);
+                Cons64::to_prim(Us, W, gs, ri);
                 lmax = nanmax(lmax, Cons64::wave_speed(W, g, ri));
             }
         }

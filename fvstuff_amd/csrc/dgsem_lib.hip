@@ -473,6 +473,7 @@ static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd, hipStream
     a.h = c->h; a.gamma = f.gamma; a.ca = s.ca; a.cb = s.cb; a.cfl = s.cfl; a.t_end = s.t_end;
     a.av_cav = f.av_cav; a.av_k = f.av_ksensor; a.eps_rho = f.eps_rho; a.eps_p = f.eps_p;
     a.off_U = L.off_U; a.off_av = L.off_av; a.off_lam = L.off_lam; a.off_el = L.off_el;
+    set_derived(a);
     const int grid = (c->NEL + E - 1) / E;
     const int block = std::min(BLOCK, (E * 2 * c->N + 63) / 64 * 64);
     HIPCK(c, stage_launcher(f.p)(a, fp32, full, grid, block, (size_t)L.total, stream));
@@ -604,11 +605,15 @@ int dg_create(const dg_config* cfg, void* stream, dg_ctx** out) {
     c->E = cfg->elems_per_block > 0 ? std::min(cfg->elems_per_block, Emax) : default_E(c->N, c->NEL);
     c->xcd = cfg->xcd_remap >= 0 ? (cfg->xcd_remap != 0) : 1;
 
-    // tables: D[N2], w[N], winv[N]
-    std::vector<double> t64(c->N2 + 2 * c->N);
-    std::vector<float> t32(c->N2 + 2 * c->N);
+    // tables: D[N2], w[N], winv[N], Dv[N2]; Dv is the volume matrix, D scaled
+    // by the volume term's factor -2/h (collocation) or -4/h (the split form's
+    // 2 D F#), so that the line loop needs no scaling pass
+    std::vector<double> t64(2 * c->N2 + 2 * c->N);
+    std::vector<float> t32(2 * c->N2 + 2 * c->N);
+    const double vscale = (cfg->volume_flux == DG_VOL_COLLOCATION ? -1.0 : -2.0) * (2.0 / c->h);
     for (int i = 0; i < c->N2; i++) t64[i] = c->D[i];
     for (int i = 0; i < c->N; i++) { t64[c->N2 + i] = c->w[i]; t64[c->N2 + c->N + i] = 1.0 / c->w[i]; }
+    for (int i = 0; i < c->N2; i++) t64[c->N2 + 2 * c->N + i] = vscale * c->D[i];
     for (size_t i = 0; i < t64.size(); i++) t32[i] = (float)t64[i];
 
     const size_t S = (size_t)c->NEL * c->N2;
@@ -988,6 +993,7 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
             a.cfl = cfl[i]; a.t_end = t_end;
             a.av_cav = f.av_cav; a.av_k = f.av_ksensor; a.eps_rho = f.eps_rho; a.eps_p = f.eps_p;
             a.off_U = L.off_U; a.off_av = L.off_av; a.off_lam = L.off_lam; a.off_el = L.off_el;
+            set_derived(a);
             b->h_args[(size_t)i * 3 + (s - 1)] = a;
         }
     }

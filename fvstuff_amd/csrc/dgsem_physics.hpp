@@ -49,6 +49,14 @@ template <typename Real> struct Prim {
     Real r, u, v, p, z;
 };
 
+// Gas constants of the FP64 state side: gamma, gamma - 1 and the energy
+// floor 1e-14/(gamma - 1) (a kernel can take them precomputed, or derive them
+// with `of`; both give the same bits).
+struct Gas64 {
+    double g, gm1, efloor;
+    static __device__ __forceinline__ Gas64 of(double g) { return Gas64{g, g - 1.0, 1e-14 / (g - 1.0)}; }
+};
+
 // ── FP64 state-side helpers (sanitize + prim conversion), :208-229, :347-355 ──
 struct Cons64 {
     // 1/x for x in the normal range used here ([1e-14, 1e30]): the hardware
@@ -61,36 +69,47 @@ struct Cons64 {
         e = fma(-x, y, 1.0);
         return fma(y, e, y);
     }
-    static __device__ __forceinline__ double pressure_from_cons(const double U[4], double g) {
+    static __device__ __forceinline__ double pressure_from_cons(const double U[4], const Gas64& gs) {
         double rho = fmax(U[0], 1e-14);
         double ke = 0.5 * (U[1] * U[1] + U[2] * U[2]) * rcp(rho);
-        return (g - 1.0) * (U[3] - ke);
+        return gs.gm1 * (U[3] - ke);
     }
-    static __device__ __forceinline__ void sanitize_cons(double U[4], double g) {
+    static __device__ __forceinline__ void sanitize_cons(double U[4], const Gas64& gs) {
         U[0] = fmax(U[0], 1e-14);
         double ke = 0.5 * (U[1] * U[1] + U[2] * U[2]) * rcp(U[0]);
-        if ((g - 1.0) * (U[3] - ke) < 1e-14) U[3] = 1e-14 / (g - 1.0) + ke;
+        if (gs.gm1 * (U[3] - ke) < 1e-14) U[3] = gs.efloor + ke;
     }
     // cons_to_prim_sane: sanitize, convert, floor rho/p, clamp |u|,|v| <= 1e4
-    // (one reciprocal of rho serves u, v and the kinetic energy)
-    static __device__ __forceinline__ void to_prim(const double U[4], double W[4], double g) {
-        double ri;
-        to_prim(U, W, g, ri);
-    }
-    // the same, also returning ri = 1/W[0]
-    static __device__ __forceinline__ void to_prim(const double U[4], double W[4], double g, double& ri) {
+    // (one reciprocal of rho serves u, v and the kinetic energy); also
+    // returns ri = 1/W[0]
+    static __device__ __forceinline__ void to_prim(const double U[4], double W[4], const Gas64& gs, double& ri) {
         const double r = fmax(U[0], 1e-14);
         ri = rcp(r);
         const double ke = 0.5 * (U[1] * U[1] + U[2] * U[2]) * ri;
         double E = U[3];
-        if ((g - 1.0) * (E - ke) < 1e-14) E = 1e-14 / (g - 1.0) + ke;
+        if (gs.gm1 * (E - ke) < 1e-14) E = gs.efloor + ke;
         W[0] = r;
         W[1] = U[1] * ri;
         W[2] = U[2] * ri;
-        W[3] = fmax((g - 1.0) * (E - ke), 1e-14);
+        W[3] = fmax(gs.gm1 * (E - ke), 1e-14);
         if (W[0] < 1e-5) { W[1] = 0.0; W[2] = 0.0; return; }
         W[1] = fmax(fmin(W[1], 1e4), -1e4);
         W[2] = fmax(fmin(W[2], 1e4), -1e4);
+    }
+    static __device__ __forceinline__ void to_prim(const double U[4], double W[4], const Gas64& gs) {
+        double ri;
+        to_prim(U, W, gs, ri);
+    }
+    // the same with gamma only
+    static __device__ __forceinline__ double pressure_from_cons(const double U[4], double g) {
+        return pressure_from_cons(U, Gas64::of(g));
+    }
+    static __device__ __forceinline__ void sanitize_cons(double U[4], double g) { sanitize_cons(U, Gas64::of(g)); }
+    static __device__ __forceinline__ void to_prim(const double U[4], double W[4], double g) {
+        to_prim(U, W, Gas64::of(g));
+    }
+    static __device__ __forceinline__ void to_prim(const double U[4], double W[4], double g, double& ri) {
+        to_prim(U, W, Gas64::of(g), ri);
     }
     // max wave speed |u|+|v|+c of sanitized primitives (:483-486)
     static __device__ __forceinline__ double wave_speed(const double W[4], double g) {
@@ -132,9 +151,6 @@ template <typename Real> struct Phys {
         F[1] = w.r * un * w.u + (DIR == 0 ? w.p : (Real)0);
         F[2] = w.r * un * w.v + (DIR == 1 ? w.p : (Real)0);
         F[3] = (E + w.p) * un;
-    }
-    __device__ __forceinline__ void flux_dir(int dir, const Prim<Real>& w, Real F[4]) const {
-        if (dir == 0) flux<0>(w, F); else flux<1>(w, F);
     }
 
     // Log mean L(a,b) = (a-b)/ln(a/b) and its inverse.  With f = (a-b)/(a+b),
@@ -258,20 +274,28 @@ template <typename Real> struct Phys {
         }
     }
 
-    // Rusanov: 1/2 (fL + fR) - 1/2 lam (UR - UL), lam = max(|un| + c).
-    __device__ __forceinline__ void rusanov(int dir, const Prim<Real>& L, const Prim<Real>& R, Real F[4]) const {
-        Real FL[4], FR[4];
-        flux_dir(dir, L, FL);
-        flux_dir(dir, R, FR);
-        Real unL = dir == 0 ? L.u : L.v, unR = dir == 0 ? R.u : R.v;
-        Real lam = Mx::fmax_(Mx::abs_(unL) + sound_speed(L.p, L.r), Mx::abs_(unR) + sound_speed(R.p, R.r));
-        Real EL = L.p * inv_gm1 + (Real)0.5 * L.r * (L.u * L.u + L.v * L.v);
-        Real ER = R.p * inv_gm1 + (Real)0.5 * R.r * (R.u * R.u + R.v * R.v);
-        Real hl = (Real)0.5 * lam;
-        F[0] = (Real)0.5 * (FL[0] + FR[0]) - hl * (R.r - L.r);
-        F[1] = (Real)0.5 * (FL[1] + FR[1]) - hl * (R.r * R.u - L.r * L.u);
-        F[2] = (Real)0.5 * (FL[2] + FR[2]) - hl * (R.r * R.v - L.r * L.v);
-        F[3] = (Real)0.5 * (FL[3] + FR[3]) - hl * (ER - EL);
+    // Rusanov flux along x, F*(L,R) = 1/2 (f_L + f_R) - lam/2 (U_R - U_L) with
+    // lam = max(|u| + c), seen from one side of a face: `own` is the left
+    // state (own_left) or the right one, n the other.  Returns F*(L,R) (JUMP
+    // false) or F*(L,R) - f(own) (JUMP true), written as
+    //   1/2 (f_own + f_nb) - s lam/2 (U_nb - U_own)   or
+    //   1/2 (f_nb - f_own) - s lam/2 (U_nb - U_own),   s = +1 (own left), -1,
+    // with no left/right selection of the states.  Both sides of a face compute F* with the same
+    // bits (the sums are commutative and U_nb - U_own changes sign exactly).
+    template <bool JUMP>
+    __device__ __forceinline__ void rusanov_side(const Prim<Real>& o, const Prim<Real>& n, bool own_left,
+                                                 Real d[4]) const {
+        const Real Eo = o.p * inv_gm1 + (Real)0.5 * o.r * (o.u * o.u + o.v * o.v);
+        const Real En = n.p * inv_gm1 + (Real)0.5 * n.r * (n.u * n.u + n.v * n.v);
+        const Real mo = o.r * o.u, mn = n.r * n.u;        // mass flux = x-momentum
+        const Real lam = Mx::fmax_(Mx::abs_(o.u) + sound_speed(o.p, o.r), Mx::abs_(n.u) + sound_speed(n.p, n.r));
+        const Real hl = own_left ? (Real)0.5 * lam : (Real)-0.5 * lam;
+        const Real fo[4] = {mo, mo * o.u + o.p, mo * o.v, (Eo + o.p) * o.u};
+        const Real fn[4] = {mn, mn * n.u + n.p, mn * n.v, (En + n.p) * n.u};
+        const Real du[4] = {n.r - o.r, mn - mo, n.r * n.v - o.r * o.v, En - Eo};
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            d[q] = (Real)0.5 * (JUMP ? fn[q] - fo[q] : fo[q] + fn[q]) - hl * du[q];
     }
 };
 
