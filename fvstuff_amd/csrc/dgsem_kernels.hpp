@@ -366,8 +366,15 @@ template <int P, int VOL, bool FULL> struct Occ<P, double, VOL, FULL> {
 // One tile (E consecutive elements) of one stage.  `lead` marks the one tile
 // per launch that updates the device step state.  Every early exit is taken
 // by the whole block (decided by thread 0, broadcast through LDS).
-template <int P, typename Real, bool FULL, int VOL, bool PIN = (sizeof(Real) == 8)>
+//
+// UP_LDS: the FP64 stage input is kept in LDS from the node phase to the
+// update (required by the limiter); otherwise it is loaded again for the
+// update, which takes more than half of the LDS of a workgroup away (the
+// batched kernel, whose large workgroups are LDS-limited, runs faster so; the
+// single-solve kernels run slower, measured).
+template <int P, typename Real, bool FULL, int VOL, bool PIN = (sizeof(Real) == 8), bool UP_LDS = true>
 __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int tile, bool lead) {
+    static_assert(UP_LDS || !FULL, "the limiter needs the stage values in LDS");
     constexpr int N = P + 1, N2 = N * N;
     using Mx = M<Real>;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -540,7 +547,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         const int e = s / N2, nd = s - e * N2;
         double W[4] = {1.0, 0.0, 0.0, 1.0};
         if (s < nE * N2) {
-            if (staged) {
+            if (UP_LDS && staged) {
 #pragma unroll
                 for (int q = 0; q < 4; q++) sU[e * 4 * N2 + q * N2 + nd] = Ul[j][q];
             }
@@ -695,9 +702,20 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                 }
         }
     }
-    // U^n for the update (stages 2, 3), issued now so that the loads are in
-    // flight across the barriers and the LDS exchange below
-    double U0l[JMAX][4];
+    // U^n for the update (stages 2, 3) and, without UP_LDS, the stage input
+    // again, issued now so that the loads are in flight across the
+    // barriers and the LDS exchange below
+    double U0l[JMAX][4], Upl[JMAX][4];
+    if (!UP_LDS && staged) {
+#pragma unroll
+        for (int j = 0; j < JMAX; j++) {
+            const int s = tid + j * nthr;
+            if (s < nE * N2) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) Upl[j][q] = ld_at(upq[q], (unsigned)s * 8u);
+            }
+        }
+    }
     if (staged && a.ca != 0.0) {
         gcptr u0q[4];
 #pragma unroll
@@ -758,13 +776,16 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         const int e = s / N2, nd = s - e * N2;
         const Real* r = sR + e * 4 * N2 + nd;
         double* up = sU + e * 4 * N2 + nd;
+        double Ui[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) Ui[q] = UP_LDS ? up[q * N2] : Upl[j][q];
         double Us[4];
         if (a.ca != 0.0) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) Us[q] = a.ca * U0l[j][q] + a.cb * (up[q * N2] + dt * (double)r[q * N2]);
+            for (int q = 0; q < 4; q++) Us[q] = a.ca * U0l[j][q] + a.cb * (Ui[q] + dt * (double)r[q * N2]);
         } else {
 #pragma unroll
-            for (int q = 0; q < 4; q++) Us[q] = up[q * N2] + dt * (double)r[q * N2];
+            for (int q = 0; q < 4; q++) Us[q] = Ui[q] + dt * (double)r[q * N2];
         }
         Cons64::sanitize_cons(Us, gs);
         if (lim) {
@@ -813,8 +834,15 @@ __global__ __launch_bounds__(BLOCK, (Occ<P, Real, VOL, FULL>::waves)) void stage
 // costs ~70 registers and spills).  args[3*i + stage-1] holds instance i's
 // stage arguments (its own device step state and wave-speed slots).  The
 // kernel carries every degree's body; with the accumulator pin its register
-// budget fits 4 waves/SIMD in FP32 (2 in FP64).
-template <typename Real> struct BatchOcc { static constexpr int waves = sizeof(Real) == 4 ? 2 : 1; };
+// budget fits 4 waves/SIMD in FP32 (2 in FP64).  In FP32 the stage input is
+// loaded again for the update instead of being kept in LDS (UP_LDS false):
+// the LDS of its up-to-256-lane workgroups would otherwise limit the
+// occupancy (11 % faster); in FP64 registers limit it first and keeping the
+// input in LDS is faster (measured).
+template <typename Real> struct BatchOcc {
+    static constexpr int waves = sizeof(Real) == 4 ? 2 : 1;
+    static constexpr bool up_lds = sizeof(Real) == 8;   // stage input kept in LDS (FP64 only)
+};
 
 template <typename Real, bool FULL, int VOL>
 __global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_kernel(
@@ -825,13 +853,13 @@ __global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_ke
     const KArgs& a = args[(size_t)tl.x * 3 + (stage - 1)];
     const bool lead = tl.y == 0;
     switch (a.P) {
-        case 1: stage_tile<1, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
-        case 2: stage_tile<2, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
-        case 3: stage_tile<3, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
-        case 4: stage_tile<4, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
-        case 5: stage_tile<5, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
-        case 6: stage_tile<6, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
-        default: stage_tile<7, Real, FULL, VOL, true>(a, sc, tl.y, lead); break;
+        case 1: stage_tile<1, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
+        case 2: stage_tile<2, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
+        case 3: stage_tile<3, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
+        case 4: stage_tile<4, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
+        case 5: stage_tile<5, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
+        case 6: stage_tile<6, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
+        default: stage_tile<7, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
     }
 }
 

@@ -973,7 +973,7 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
         const dg_config& f = c->cfg;
         const bool fp32 = f.precision == DG_MIXED;
         const int E = batch_elems_per_block(c->N, b->lanes);
-        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, true, false, false);
+        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, !fp32, false, false);   // FP64: input in LDS
         for (int s = 1; s <= 3; s++) {
             KArgs a;
             memset(&a, 0, sizeof(a));
@@ -1012,7 +1012,8 @@ static hipError_t batch_tiles(dg_batch* b) {
         const int nt = (c->NEL + E - 1) / E;
         const double cost = (double)E * c->N * c->N * c->N;
         for (int t = 0; t < nt; t++) tl.push_back({cost, make_int2(i, t)});
-        LdsLayout L = lds_layout(c->N, E, b->common.precision == DG_MIXED ? 4 : 8, true, false, false);
+        const bool fp32 = b->common.precision == DG_MIXED;
+        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, !fp32, false, false);
         lds = std::max(lds, (size_t)L.total);
     }
     std::stable_sort(tl.begin(), tl.end(), [](const std::pair<double, int2>& x, const std::pair<double, int2>& y) {
