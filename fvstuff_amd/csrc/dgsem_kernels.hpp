@@ -372,7 +372,9 @@ template <int P, int VOL, bool FULL> struct Occ<P, double, VOL, FULL> {
 // update, which takes more than half of the LDS of a workgroup away (the
 // batched kernel, whose large workgroups are LDS-limited, runs faster so; the
 // single-solve kernels run slower, measured).
-template <int P, typename Real, bool FULL, int VOL, bool PIN = (sizeof(Real) == 8), bool UP_LDS = true>
+// U0_EARLY: U^n is loaded with the stage input at the start (see below).
+template <int P, typename Real, bool FULL, int VOL, bool PIN = (sizeof(Real) == 8), bool UP_LDS = true,
+          bool U0_EARLY = !FULL>
 __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int tile, bool lead) {
     static_assert(UP_LDS || !FULL, "the limiter needs the stage values in LDS");
     constexpr int N = P + 1, N2 = N * N;
@@ -459,7 +461,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     // conversion, so each wave keeps many requests in flight.
     constexpr int JMAX = (N + 1) / 2;
     const bool staged = a.mode == MODE_STAGE;
-    double Ul[JMAX][4];
+    double Ul[JMAX][4], U0l[JMAX][4], Upl[JMAX][4];
     gcptr upq[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) upq[q] = sbase(a.Up + tile_off + q * S);
@@ -471,6 +473,26 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             for (int q = 0; q < 4; q++) Ul[j][q] = ld_at(upq[q], (unsigned)s * 8u);
         }
     }
+    // U^n for the update (stages 2, 3).  With U0_EARLY (the lean kernels) it
+    // is issued with the stage input, so that the update never waits for
+    // memory (32 more registers, within their occupancy target; measured 2-4 %
+    // faster); the shock-capturing kernels, whose AV and limiter code needs
+    // the registers, and the FP32 batched kernel, whose occupancy they would
+    // lower, issue it after the line phase.
+    auto load_u0 = [&]() {
+        gcptr u0q[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) u0q[q] = sbase(a.U0 + tile_off + q * S);
+#pragma unroll
+        for (int j = 0; j < JMAX; j++) {
+            const int s = tid + j * nthr;
+            if (s < nE * N2) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) U0l[j][q] = ld_at(u0q[q], (unsigned)s * 8u);
+            }
+        }
+    };
+    if (U0_EARLY && staged && a.ca != 0.0) load_u0();
     // ── fused dt (device side, :1026-1035) ──
     // Decided after the tile's loads are issued (they are in bounds either
     // way), so the step-state read and the loads overlap.
@@ -702,10 +724,10 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                 }
         }
     }
-    // U^n for the update (stages 2, 3) and, without UP_LDS, the stage input
-    // again, issued now so that the loads are in flight across the
-    // barriers and the LDS exchange below
-    double U0l[JMAX][4], Upl[JMAX][4];
+    // U^n (shock-capturing kernels) and, without UP_LDS, the stage input
+    // again, issued now so that the loads are in flight across the barriers
+    // and the LDS exchange below
+    if (!U0_EARLY && staged && a.ca != 0.0) load_u0();
     if (!UP_LDS && staged) {
 #pragma unroll
         for (int j = 0; j < JMAX; j++) {
@@ -713,19 +735,6 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             if (s < nE * N2) {
 #pragma unroll
                 for (int q = 0; q < 4; q++) Upl[j][q] = ld_at(upq[q], (unsigned)s * 8u);
-            }
-        }
-    }
-    if (staged && a.ca != 0.0) {
-        gcptr u0q[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) u0q[q] = sbase(a.U0 + tile_off + q * S);
-#pragma unroll
-        for (int j = 0; j < JMAX; j++) {
-            const int s = tid + j * nthr;
-            if (s < nE * N2) {
-#pragma unroll
-                for (int q = 0; q < 4; q++) U0l[j][q] = ld_at(u0q[q], (unsigned)s * 8u);
             }
         }
     }
@@ -838,10 +847,12 @@ __global__ __launch_bounds__(BLOCK, (Occ<P, Real, VOL, FULL>::waves)) void stage
 // loaded again for the update instead of being kept in LDS (UP_LDS false):
 // the LDS of its up-to-256-lane workgroups would otherwise limit the
 // occupancy (11 % faster); in FP64 registers limit it first and keeping the
-// input in LDS is faster (measured).
+// input in LDS is faster, as is loading U^n at the start (5 %; in FP32 that
+// costs occupancy and 13 %, measured).
 template <typename Real> struct BatchOcc {
     static constexpr int waves = sizeof(Real) == 4 ? 2 : 1;
-    static constexpr bool up_lds = sizeof(Real) == 8;   // stage input kept in LDS (FP64 only)
+    static constexpr bool up_lds = sizeof(Real) == 8;     // stage input kept in LDS (FP64 only)
+    static constexpr bool u0_early = sizeof(Real) == 8;   // U^n loaded at the start (FP64 only)
 };
 
 template <typename Real, bool FULL, int VOL>
@@ -853,13 +864,20 @@ __global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_ke
     const KArgs& a = args[(size_t)tl.x * 3 + (stage - 1)];
     const bool lead = tl.y == 0;
     switch (a.P) {
-        case 1: stage_tile<1, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
-        case 2: stage_tile<2, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
-        case 3: stage_tile<3, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
-        case 4: stage_tile<4, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
-        case 5: stage_tile<5, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
-        case 6: stage_tile<6, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
-        default: stage_tile<7, Real, FULL, VOL, true, BatchOcc<Real>::up_lds>(a, sc, tl.y, lead); break;
+        case 1: stage_tile<1, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
+                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
+        case 2: stage_tile<2, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
+                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
+        case 3: stage_tile<3, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
+                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
+        case 4: stage_tile<4, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
+                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
+        case 5: stage_tile<5, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
+                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
+        case 6: stage_tile<6, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
+                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
+        default: stage_tile<7, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
+                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
     }
 }
 
