@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per stage launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of the
 # stage kernel (scaled by a device copy's readings), keyed by (K, p, form);
 # the runs and the scaling are listed in the named profile.
-PMC_TRAFFIC = {(512, 7, "mixed"): (1.457e9, "profiles/stage_kernel_p7_vortex512.md")}
+PMC_TRAFFIC = {(512, 7, "mixed"): (1.437e9, "profiles/stage_kernel_p7_vortex512.md")}
 FP64_PEAK_TFLOPS = 78.6
 
 

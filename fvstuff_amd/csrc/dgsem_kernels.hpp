@@ -276,7 +276,7 @@ __device__ __forceinline__ void rot(Prim<Real>& w, bool swap) {
 // every pair's flux and accumulation past the later pairs' log-branches into
 // one block at the end of the line, keeping all pairs' intermediate values
 // live at once: in FP64 at p = 7 that needs the whole register file (1 wave
-// per SIMD) instead of 218 registers (2 waves).  The FP32 single-solve kernel
+// per SIMD) instead of about 220 registers (2 waves).  The FP32 single-solve kernel
 // keeps the sunk schedule, whose wide independent block runs faster at its
 // occupancy (measured); the batched kernel pins (see batch_stage_kernel).
 template <int N, bool PIN, typename Real>
