@@ -249,6 +249,44 @@ def test_end_time_and_status_codes(dgsem):
     assert e.value.status == 1
 
 
+RAGGED_CASES = [
+    # K = 7: 49 elements, so every tile size below leaves a partial last tile
+    ("vortex-p3", lambda E: vortex_config(K=7, p=3, elems_per_block=E), "vortex", 0.5),
+    ("vortex-p7", lambda E: vortex_config(K=7, p=7, elems_per_block=E), "vortex", 0.2),
+    ("kh-shock-capturing-p3", lambda E: kh_config(K=7, p=3, precision="fp64", volume_flux="chandrashekar",
+                                                  surface_flux="hllc", av=True, limiter=True,
+                                                  elems_per_block=E), "kh", 0.5),
+]
+
+
+@pytest.mark.parametrize("name,mk,ic,cfl", RAGGED_CASES, ids=[c[0] for c in RAGGED_CASES])
+def test_ragged_tiles(dgsem, name, mk, ic, cfl):
+    """Tiles of 1, 3, 4, 5 and the largest number of elements over a 7x7
+    grid: the RHS matches the oracle for every tiling (slots past the last
+    element read a stand-in node and write nothing), and 4 fused steps give
+    the same bits for every tiling (a face sees the same neighbour state
+    whether it comes from LDS or from global memory)."""
+    cfg0 = mk(0)
+    o = Oracle(cfg0)
+    Q0 = o.init_state(make_ic(ic, noise_amp=0.3) if ic == "kh" else make_ic(ic))
+    R_ref, lam_ref = o.compute_rhs(Q0)
+    Emax = 256 // (2 * (cfg0.p + 1))
+    ref = None
+    for E in sorted({1, 3, 4, 5, Emax}):
+        s = dgsem.Solver(mk(E))
+        R, lam = s.compute_rhs(gpu(Q0))
+        assert rel_l2(cpu(R), R_ref, w2(cfg0)) < 1e-13, E
+        np.testing.assert_allclose(cpu(lam), lam_ref, rtol=1e-15)
+        Q = gpu(Q0)
+        s.advance(Q, 4, cfl)
+        q = cpu(Q)
+        assert np.all(np.isfinite(q))
+        if ref is None:
+            ref = q
+        else:
+            assert np.array_equal(q, ref), E
+
+
 def test_autotune_keeps_results(dgsem):
     cfg = vortex_config(K=40, p=3, precision="mixed")
     s = dgsem.Solver(cfg)
