@@ -170,6 +170,35 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// Wave-wide unsigned max through DPP (no LDS round trips): row shifts give
+// lane 15 of every row its row's maximum, two row broadcasts carry it to lane
+// 63.  Lanes whose DPP source lies outside the row keep their own value.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned long long umax_dpp_step(unsigned long long v) {
+    const unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+    const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp((int)lo, (int)lo, CTRL, ROW_MASK, 0xf, false);
+    const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp((int)hi, (int)hi, CTRL, ROW_MASK, 0xf, false);
+    const unsigned long long o = ((unsigned long long)ohi << 32) | olo;
+    return o > v ? o : v;
+}
+// the maximum over the 64 lanes, valid in lane 63
+__device__ __forceinline__ unsigned long long wave_umax_to_last(unsigned long long v) {
+    v = umax_dpp_step<0x111, 0xf>(v);   // row_shr:1
+    v = umax_dpp_step<0x112, 0xf>(v);   // row_shr:2
+    v = umax_dpp_step<0x114, 0xf>(v);   // row_shr:4
+    v = umax_dpp_step<0x118, 0xf>(v);   // row_shr:8
+    v = umax_dpp_step<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v = umax_dpp_step<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+    return v;
+}
+// the maximum over the 64 lanes, in every lane
+__device__ __forceinline__ unsigned long long wave_umax(unsigned long long v) {
+    v = wave_umax_to_last(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 // Block-wide max of v (FP64) -> one atomicMax on the bit pattern.
 __device__ __forceinline__ void block_max_atomic(double v, unsigned long long* dst, double* sred) {
     v = wave_max(v);
@@ -417,7 +446,6 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     double* sBar = sEl;                              // [E][4]
     double* sTrho = sEl + 4 * E;                     // [E]
     unsigned long long* sTheta = (unsigned long long*)(sEl + 5 * E);  // [E]
-    double* sred = sEl + 6 * E;                      // [BLOCK/64]
 
     const bool need_lam = FULL && (a.av || (a.mode == MODE_RHS && a.lam_el != nullptr));
 
@@ -514,17 +542,11 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     // way), so the step-state read and the loads overlap.
     double dt = 0.0;
     if (a.mode == MODE_STAGE) {
-        __syncthreads();   // a persistent loop's previous tile is done with s_go / s_dt / LDS
         unsigned long long lam_b = 0ull;
         if (a.stage == 1 && tid < 64) {
-            // max over the partial slots (non-negative doubles and +NaN order
-            // as their unsigned bit patterns)
-            lam_b = a.st->lam_part[sc.slot_cur][tid & (LAM_PARTS - 1)];
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const unsigned long long v = __shfl_xor(lam_b, o, 64);
-                lam_b = v > lam_b ? v : lam_b;
-            }
+            // max over the partial slots (non-negative doubles and NaN order
+            // as their unsigned bit patterns, NaN above every number)
+            lam_b = wave_umax(a.st->lam_part[sc.slot_cur][tid & (LAM_PARTS - 1)]);
         }
         if (tid == 0) {
             DevState* st = a.st;
@@ -843,7 +865,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             }
         }
     }
-    if (want_lambda) block_max_atomic(lmax, &a.st->lam_part[sc.slot_next][blockIdx.x & (LAM_PARTS - 1)], sred);
+    if (want_lambda) {
+        // one atomic per wave on the ordered bit pattern (the speeds are
+        // non-negative; a NaN, of either sign, orders above every number)
+        const unsigned long long m = wave_umax_to_last(dbits(lmax));
+        if ((tid & 63) == 63) atomicMax(&a.st->lam_part[sc.slot_next][blockIdx.x & (LAM_PARTS - 1)], m);
+    }
 }
 
 template <int P, typename Real, bool FULL, int VOL>
@@ -916,7 +943,7 @@ inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, b
     L.off_lam = off;
     if (lam || av) off += align_up(2 * E * 8, 16);
     L.off_el = off;
-    off += align_up((6 * E + BLOCK / 64) * 8, 16);
+    off += align_up(6 * E * 8, 16);
     L.total = off;
     return L;
 }
