@@ -415,8 +415,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     constexpr int N = P + 1, N2 = N * N;
     using Mx = M<Real>;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double s_dt;
-    __shared__ int s_go;
+    // dt and the go flag go from thread 0 to the block through the first
+    // bytes of the prims region, which the node phase writes only after every
+    // wave has read them (multi-wave blocks wait at a barrier; one wave reads
+    // before it writes)
+    double* s_dt = (double*)smem;
+    int* s_go = (int*)(smem + 8);
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int E = a.E;
     const int el0 = tile * E;
@@ -586,12 +590,14 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                     }
                 }
             }
-            s_go = go;
-            s_dt = d;
+            *s_go = go;
+            *s_dt = d;
         }
         __syncthreads();
-        if (!s_go) return;
-        dt = s_dt;
+        const int go = *s_go;
+        dt = *s_dt;
+        if (nthr > 64) __syncthreads();
+        if (!go) return;
     }
 
     if (FULL && (need_lam || a.av)) {
@@ -931,8 +937,9 @@ struct LdsLayout {
 
 inline int align_up(int x, int a) { return (x + a - 1) / a * a; }
 
-// `stage`: the FP64 input state is staged in LDS (sU) for the update phase.
-inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, bool lam) {
+// `stage`: the FP64 input state is staged in LDS (sU) for the update phase;
+// `lim`: the limiter's per-element scratch.
+inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, bool lam, bool lim) {
     const int N2 = N * N;
     LdsLayout L;
     int off = align_up(E * N2 * 5 * real_bytes, 16);
@@ -943,7 +950,7 @@ inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, b
     L.off_lam = off;
     if (lam || av) off += align_up(2 * E * 8, 16);
     L.off_el = off;
-    off += align_up(6 * E * 8, 16);
+    if (lim) off += align_up(6 * E * 8, 16);
     L.total = off;
     return L;
 }

@@ -459,7 +459,7 @@ static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd, hipStream
     const bool lam = s.mode == MODE_RHS && s.lam_el != nullptr;
     const bool lim = s.mode == MODE_STAGE && f.limiter;
     const bool full = f.av || lim || lam;
-    LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, s.mode == MODE_STAGE, f.av != 0, lam);
+    LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, s.mode == MODE_STAGE, f.av != 0, lam, lim);
     KArgs a;
     memset(&a, 0, sizeof(a));
     a.Up = s.Up; a.U0 = s.U0; a.out = s.out; a.lam_el = s.lam_el;
@@ -973,7 +973,7 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
         const dg_config& f = c->cfg;
         const bool fp32 = f.precision == DG_MIXED;
         const int E = batch_elems_per_block(c->N, b->lanes);
-        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, !fp32, false, false);   // FP64: input in LDS
+        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, !fp32, false, false, false);   // FP64: input in LDS
         for (int s = 1; s <= 3; s++) {
             KArgs a;
             memset(&a, 0, sizeof(a));
@@ -1013,7 +1013,7 @@ static hipError_t batch_tiles(dg_batch* b) {
         const double cost = (double)E * c->N * c->N * c->N;
         for (int t = 0; t < nt; t++) tl.push_back({cost, make_int2(i, t)});
         const bool fp32 = b->common.precision == DG_MIXED;
-        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, !fp32, false, false);
+        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, !fp32, false, false, false);
         lds = std::max(lds, (size_t)L.total);
     }
     std::stable_sort(tl.begin(), tl.end(), [](const std::pair<double, int2>& x, const std::pair<double, int2>& y) {
