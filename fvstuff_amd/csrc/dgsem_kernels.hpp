@@ -199,6 +199,32 @@ __device__ __forceinline__ unsigned long long wave_umax(unsigned long long v) {
     return ((unsigned long long)hi << 32) | lo;
 }
 
+// Element maximum into LDS for node slots that are element-contiguous across
+// lanes (slot s = element * N2 + node): when an element's N2 nodes fill whole
+// DPP rows (N2 = 4, 16) or the whole wave (N2 = 64), the group is reduced
+// with DPP and one lane per element issues the LDS atomic (nodes of one
+// element would otherwise serialise on the same address); other degrees use
+// one atomic per node.  All lanes of the wave must take part; `live` is
+// uniform over an element's group.
+template <int N2>
+__device__ __forceinline__ void elem_max_lds(unsigned long long* dst, unsigned long long v, bool live) {
+    const int lane = threadIdx.x & 63;
+    if (N2 == 64) {
+        v = wave_umax_to_last(v);
+        if (live && lane == 63) atomicMax(dst, v);
+    } else if (N2 == 16 || N2 == 4) {
+        v = umax_dpp_step<0x111, 0xf>(v);
+        v = umax_dpp_step<0x112, 0xf>(v);
+        if (N2 == 16) {
+            v = umax_dpp_step<0x114, 0xf>(v);
+            v = umax_dpp_step<0x118, 0xf>(v);
+        }
+        if (live && (lane & (N2 - 1)) == N2 - 1) atomicMax(dst, v);
+    } else if (live) {
+        atomicMax(dst, v);
+    }
+}
+
 // Block-wide max of v (FP64) -> one atomicMax on the bit pattern.
 __device__ __forceinline__ void block_max_atomic(double v, unsigned long long* dst, double* sred) {
     v = wave_max(v);
@@ -623,7 +649,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         Real* p = sW + s * 5;
         p[0] = (Real)0.5 * w.r; p[1] = (Real)0.5 * w.u; p[2] = (Real)0.5 * w.v;
         p[3] = (Real)0.5 * w.p; p[4] = (Real)0.5 * w.z;
-        if (FULL && need_lam && s < nE * N2) atomicMax(&sLamE[e], dbits(Cons64::wave_speed(W, g)));
+        if (FULL && need_lam) elem_max_lds<N2>(&sLamE[e], dbits(Cons64::wave_speed(W, g)), s < nE * N2);
     }
     __syncthreads();
 
@@ -649,14 +675,17 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         const Real dx2inv = (Real)1 / (len_p * len_p);
 #pragma unroll
         for (int j = 0; j < JMAX; j++) {
-            const int s = tid + j * nthr;
-            if (s >= nE * N2) continue;
-            const int e = s / N2;
+            // every lane takes part (the element reduction is wave-wide);
+            // slots past the tile read slot 0 and write nothing
+            const int s0 = tid + j * nthr;
+            const bool live = s0 < nE * N2;
+            const int s = live ? s0 : 0;
+            const int e = s0 / N2;
             const Real* p = sW + s * 5;
             Real rho = Mx::fmax_((Real)2 * p[0], (Real)1e-12), c2 = ph.g * ((Real)2 * p[3]) / rho;
             Real divu = Mx::fmin_(jac * (sDu[s] + sDv[s]), (Real)0), du2 = divu * divu;
             Real th = du2 / (du2 + (Real)a.av_k * c2 * dx2inv + (Real)1e-30);
-            atomicMax(&sThE[e], dbits((double)th));
+            elem_max_lds<N2>(&sThE[e < E ? e : 0], dbits((double)th), live);
         }
         __syncthreads();
         if (tid < E)
