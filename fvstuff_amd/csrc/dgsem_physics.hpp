@@ -162,6 +162,7 @@ template <typename Real> struct Phys {
     // Log means of one pair: the series form first (no log, one reciprocal
     // each), replaced by the ratio-form log only where the series does not
     // apply.  That branch is skipped by the whole wave on smooth data.
+    template <bool UNIFORM>
     __device__ __forceinline__ static void pair_means(Real aL, Real aR, Real bL, Real bR, Real& a_ln,
                                                       Real& inv_b_ln) {
         Real sa = aL + aR, sb = bL + bR;
@@ -177,8 +178,21 @@ template <typename Real> struct Phys {
             inv_b_ln = (Real)2 * rb *
                    ((Real)1 + ub * ((Real)(1.0 / 3.0) + ub * ((Real)(1.0 / 5.0) + ub * (Real)(1.0 / 7.0))));
         }
-        if (ua >= (Real)Mx::series_thr) a_ln = Mx::div(aL - aR, Mx::ln(Mx::div(aL, aR)));
-        if (ub >= (Real)Mx::series_thr) inv_b_ln = Mx::div(Mx::ln(Mx::div(bL, bR)), bL - bR);
+        const bool la = ua >= (Real)Mx::series_thr, lb = ub >= (Real)Mx::series_thr;
+        if (UNIFORM) {
+            // wave-uniform branch, skipped by the whole wave on smooth data
+            // (per-lane branches are if-converted: every pair then evaluates
+            // both logs); each lane selects its own value
+            if (__any(la || lb)) {
+                const Real a_ex = Mx::div(aL - aR, Mx::ln(Mx::div(aL, aR)));
+                const Real b_ex = Mx::div(Mx::ln(Mx::div(bL, bR)), bL - bR);
+                a_ln = la ? a_ex : a_ln;
+                inv_b_ln = lb ? b_ex : inv_b_ln;
+            }
+        } else {
+            if (la) a_ln = Mx::div(aL - aR, Mx::ln(Mx::div(aL, aR)));
+            if (lb) inv_b_ln = Mx::div(Mx::ln(Mx::div(bL, bR)), bL - bR);
+        }
     }
 
     // Ranocha flux on half-scaled node states h = (rho, u, v, p, rho/p)/2, the
@@ -225,10 +239,13 @@ template <typename Real> struct Phys {
     }
 
     // Chandrashekar EC flux (:259-285) along x, beta = rho/(2p) = z/2.
+    // UNIFORM: wave-uniform log branch (lines of up to 6 nodes; at 7 and 8
+    // nodes the branch form needs more registers than the kernels have)
+    template <bool UNIFORM = true>
     __device__ __forceinline__ void chandrashekar(const Prim<Real>& L, const Prim<Real>& R, Real F[4]) const {
         Real bL = (Real)0.5 * L.z, bR = (Real)0.5 * R.z;
         Real r_ln, inv_b_ln;
-        pair_means(L.r, R.r, bL, bR, r_ln, inv_b_ln);
+        pair_means<UNIFORM>(L.r, R.r, bL, bR, r_ln, inv_b_ln);
         // p_hat = rho_avg/(2 beta_avg) = (rL+rR)/(2 (bL+bR))
         Real p_hat = Mx::div((Real)0.5 * (L.r + R.r), bL + bR);
         Real u_av = (Real)0.5 * (L.u + R.u), v_av = (Real)0.5 * (L.v + R.v);
