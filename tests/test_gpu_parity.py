@@ -82,6 +82,10 @@ REF_CFGS = [
     # the reference's own default configuration: walls, Chandrashekar, HLLC, AV, limiter
     ("sod-reference", make_config(K=12, p=3), "sod"),
     ("sod-reference-p2", make_config(K=12, p=2), "sod"),
+    # element maxima (wave speed, AV sensor) reduced across 4-lane groups
+    # (p=1) and whole waves (p=7) before the LDS atomic
+    ("sod-reference-p1", make_config(K=12, p=1), "sod"),
+    ("sod-reference-p7", make_config(K=6, p=7), "sod"),
     ("sod-rusanov-ranocha-p5", make_config(K=8, p=5, volume_flux="ranocha", surface_flux="rusanov"), "sod"),
     ("kh-ranocha-rusanov", kh_config(K=8, p=3, precision="fp64"), "kh"),
     ("kh-chandrashekar-hllc-av", kh_config(K=8, p=4, precision="fp64", volume_flux="chandrashekar",
