@@ -763,10 +763,11 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
 
         line_volume<N, Real, VOL, PIN>(ph, hw, Dv, cm, cp, acc);
 
-        // ── artificial viscosity along the line: -jac W^-1 D^T W (nu jac D Uc) ──
+        // ── artificial viscosity along the line: -jac W^-1 D^T W (nu jac D Uc)
+        //    = -nu jac^2 Lav Uc, Lav = W^-1 D^T W D precomputed (one pass) ──
         if (FULL && a.av) {
-            const Real nu = sNu[le];
-            const Real* wq = D + N * N;
+            const Real f = -sNu[le] * jac * jac;
+            const Real* Lav = D + 2 * N * N + 2 * N;
             Real uc[N][4];      // conserved variables of the line's nodes (u, v as rotated)
 #pragma unroll
             for (int k = 0; k < N; k++) {
@@ -776,24 +777,14 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                 uc[k][2] = w.r * w.v;
                 uc[k][3] = w.p * ph.inv_gm1 + (Real)0.5 * w.r * (w.u * w.u + w.v * w.v);
             }
-            Real gr[N][4];
 #pragma unroll
             for (int i = 0; i < N; i++)
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     Real s = 0;
 #pragma unroll
-                    for (int k = 0; k < N; k++) s += D[i * N + k] * uc[k][q];
-                    gr[i][q] = nu * jac * s;
-                }
-#pragma unroll
-            for (int i = 0; i < N; i++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    Real s = 0;
-#pragma unroll
-                    for (int k = 0; k < N; k++) s += wq[k] * D[k * N + i] * gr[k][q];
-                    acc[i][q] -= jac * winv[i] * s;
+                    for (int m = 0; m < N; m++) s += Lav[i * N + m] * uc[m][q];
+                    acc[i][q] += f * s;
                 }
         }
     }

@@ -605,15 +605,23 @@ int dg_create(const dg_config* cfg, void* stream, dg_ctx** out) {
     c->E = cfg->elems_per_block > 0 ? std::min(cfg->elems_per_block, Emax) : default_E(c->N, c->NEL);
     c->xcd = cfg->xcd_remap >= 0 ? (cfg->xcd_remap != 0) : 1;
 
-    // tables: D[N2], w[N], winv[N], Dv[N2]; Dv is the volume matrix, D scaled
-    // by the volume term's factor -2/h (collocation) or -4/h (the split form's
-    // 2 D F#), so that the line loop needs no scaling pass
-    std::vector<double> t64(2 * c->N2 + 2 * c->N);
-    std::vector<float> t32(2 * c->N2 + 2 * c->N);
+    // tables: D[N2], w[N], winv[N], Dv[N2], Lav[N2].  Dv is the volume matrix,
+    // D scaled by the volume term's factor -2/h (collocation) or -4/h (the
+    // split form's 2 D F#), so that the line loop needs no scaling pass; Lav =
+    // W^-1 D^T W D is the artificial-viscosity operator of one line (the
+    // reference's two passes, D then W^-1 D^T W, in one matrix)
+    std::vector<double> t64(3 * c->N2 + 2 * c->N);
+    std::vector<float> t32(3 * c->N2 + 2 * c->N);
     const double vscale = (cfg->volume_flux == DG_VOL_COLLOCATION ? -1.0 : -2.0) * (2.0 / c->h);
     for (int i = 0; i < c->N2; i++) t64[i] = c->D[i];
     for (int i = 0; i < c->N; i++) { t64[c->N2 + i] = c->w[i]; t64[c->N2 + c->N + i] = 1.0 / c->w[i]; }
     for (int i = 0; i < c->N2; i++) t64[c->N2 + 2 * c->N + i] = vscale * c->D[i];
+    for (int i = 0; i < c->N; i++)
+        for (int m = 0; m < c->N; m++) {
+            double sum = 0.0;
+            for (int k = 0; k < c->N; k++) sum += c->w[k] * c->D[k * c->N + i] * c->D[k * c->N + m];
+            t64[2 * c->N2 + 2 * c->N + i * c->N + m] = sum / c->w[i];
+        }
     for (size_t i = 0; i < t64.size(); i++) t32[i] = (float)t64[i];
 
     const size_t S = (size_t)c->NEL * c->N2;
