@@ -637,19 +637,19 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         const int s = tid + j * nthr;
         if (s >= E * N2) continue;
         const int e = s / N2, nd = s - e * N2;
-        double W[4] = {1.0, 0.0, 0.0, 1.0};
+        double W[4] = {1.0, 0.0, 0.0, 1.0}, ri = 1.0;
         if (s < nE * N2) {
             if (UP_LDS && staged) {
 #pragma unroll
                 for (int q = 0; q < 4; q++) sU[e * 4 * N2 + q * N2 + nd] = Ul[j][q];
             }
-            Cons64::to_prim(Ul[j], W, gs);
+            Cons64::to_prim(Ul[j], W, gs, ri);
         }
         Prim<Real> w = make_prim<Real>(W);
         Real* p = sW + s * 5;
         p[0] = (Real)0.5 * w.r; p[1] = (Real)0.5 * w.u; p[2] = (Real)0.5 * w.v;
         p[3] = (Real)0.5 * w.p; p[4] = (Real)0.5 * w.z;
-        if (FULL && need_lam) elem_max_lds<N2>(&sLamE[e], dbits(Cons64::wave_speed(W, g)), s < nE * N2);
+        if (FULL && need_lam) elem_max_lds<N2>(&sLamE[e], dbits(Cons64::wave_speed(W, g, ri)), s < nE * N2);
     }
     __syncthreads();
 
