@@ -25,12 +25,14 @@ SURF_HLLC, SURF_RUSANOV = 0, 1
 BC_WALL, BC_PERIODIC = 0, 1
 FP64, MIXED = 0, 1
 IC_CIRCULAR_SOD, IC_VORTEX, IC_KELVIN_HELMHOLTZ = 0, 1, 2
+SCHEME_DGSEM, SCHEME_SD = 0, 1
 
 _NAMES = {
     "volume_flux": {"chandrashekar": 0, "ranocha": 1, "collocation": 2},
     "surface_flux": {"hllc": 0, "rusanov": 1},
     "bc": {"wall": 0, "periodic": 1},
     "precision": {"fp64": 0, "mixed": 1},
+    "scheme": {"dgsem": 0, "sd": 1},
 }
 
 
@@ -44,6 +46,7 @@ class DGConfig(C.Structure):
         ("av_cav", C.c_double), ("av_ksensor", C.c_double),
         ("eps_rho", C.c_double), ("eps_p", C.c_double),
         ("elems_per_block", C.c_int), ("xcd_remap", C.c_int),
+        ("scheme", C.c_int),
     ]
 
 
@@ -61,7 +64,7 @@ def make_config(K=50, p=3, x0=0.0, y0=0.0, L=1.0, gamma=1.4,
                 volume_flux="chandrashekar", surface_flux="hllc", bc="wall",
                 av=True, limiter=True, precision="fp64", av_cav=0.25,
                 av_ksensor=0.1, eps_rho=1e-6, eps_p=1e-6,
-                elems_per_block=0, xcd_remap=-1):
+                elems_per_block=0, xcd_remap=-1, scheme="dgsem"):
     """A dg_config; defaults are the reference's compile-time defaults
     (dgsem_lobatto_2d.cu:61-126, Makefile DGSEM_ORDER=3, main :915-922)."""
     def enum(field, v):
@@ -70,7 +73,18 @@ def make_config(K=50, p=3, x0=0.0, y0=0.0, L=1.0, gamma=1.4,
                     enum("volume_flux", volume_flux), enum("surface_flux", surface_flux),
                     enum("bc", bc), int(bool(av)), int(bool(limiter)),
                     enum("precision", precision), float(av_cav), float(av_ksensor),
-                    float(eps_rho), float(eps_p), int(elems_per_block), int(xcd_remap))
+                    float(eps_rho), float(eps_p), int(elems_per_block), int(xcd_remap),
+                    enum("scheme", scheme))
+
+
+def sd_config(K=50, p=3, precision="fp64", **kw):
+    """The spectral-difference sibling (fr_lobatto_2d.cu) with its driver's
+    defaults: circular Sod on [0,1]^2, reflecting walls, HLLC faces, nodal
+    positivity repair, no artificial viscosity (main :835-838, CFL 0.1/p)."""
+    args = dict(K=K, p=p, x0=0.0, y0=0.0, L=1.0, surface_flux="hllc", bc="wall",
+                av=False, limiter=True, precision=precision, scheme="sd")
+    args.update(kw)
+    return make_config(**args)
 
 
 def make_ic(kind="vortex", rho_hi=11.0, p_hi=10.0, vortex_eps=5.0, xc=5.0, yc=5.0,

@@ -91,6 +91,10 @@ static hipError_t launch_vol(const KArgs& a, int grid, int block, size_t lds, hi
 
 hipError_t DG_CAT(launch_stage_p, DG_P)(const KArgs& a, bool fp32, bool full, int grid, int block, size_t lds,
                                         hipStream_t s) {
+    // the spectral-difference sibling has one (lean) kernel per precision
+    if (a.vol == VOL_SD)
+        return fp32 ? launch_one<float, false, VOL_SD>(a, grid, block, lds, s)
+                    : launch_one<double, false, VOL_SD>(a, grid, block, lds, s);
     if (fp32) return full ? launch_vol<float, true>(a, grid, block, lds, s) : launch_vol<float, false>(a, grid, block, lds, s);
     return full ? launch_vol<double, true>(a, grid, block, lds, s) : launch_vol<double, false>(a, grid, block, lds, s);
 }

@@ -24,13 +24,20 @@
 //                   (:533-576) with sanitize_cons, the Zhang-Shu limiter
 //                   (:584-643, FP64) and the wave-speed reduction for the next
 //                   step's dt (:483-486, :646-659).
+//
+// The spectral-difference sibling of fr_lobatto_2d.cu (VOL_SD) shares the
+// three phases: its line thread interpolates the line's conservative states
+// to the interior flux nodes instead of evaluating two-point fluxes
+// (sd_line), and its update applies the nodal positivity repair.
 #pragma once
 #include "dgsem_physics.hpp"
 
 namespace dg {
 
 enum { MODE_RHS = 0, MODE_STAGE = 1 };
-enum { VOL_CHANDRASHEKAR = 0, VOL_RANOCHA = 1, VOL_COLLOCATION = 2 };
+// VOL_SD: the spectral-difference sibling (fr_lobatto_2d.cu), whose line
+// operator replaces the volume term
+enum { VOL_CHANDRASHEKAR = 0, VOL_RANOCHA = 1, VOL_COLLOCATION = 2, VOL_SD = 3 };
 enum { SURF_HLLC = 0, SURF_RUSANOV = 1 };
 enum { BC_WALL = 0, BC_PERIODIC = 1 };
 enum { ST_OK = 0, ST_NONFINITE = 3, ST_DT_UNDERFLOW = 4, ST_BLOWUP = 5 };
@@ -412,6 +419,49 @@ __device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Rea
     line_pairs<N, Real, VOL, PIN>(ph, hw, Dv, acc);
 }
 
+// ── spectral-difference line (fr_lobatto_2d.cu:367-438): the conservative
+//    states of the line's p+1 solution nodes (FP64 in LDS, eU[q*N2 + node])
+//    interpolated to the p interior flux nodes, converted (cons_to_prim_sane)
+//    and their physical flux evaluated; the Riemann fluxes fm, fp sit on the
+//    two end flux nodes; the flux polynomial's derivative at the solution
+//    nodes through Dsd = -(2/h) Dfs:
+//    acc[i] = Dsd[i][0] fm + Dsd[i][N] fp + sum_{m=1}^{N-1} Dsd[i][m] f(Isf[m] U)
+//    y-lines read the momenta swapped and run the x-direction code. ──
+template <int N, typename Real>
+__device__ __forceinline__ void sd_line(const Phys<Real>& ph, const double* eU, int lbase, int lstride, bool ydir,
+                                        const Real* Isf, const Real* Dsd, const Real (&fm)[4],
+                                        const Real (&fp)[4], Real (&acc)[N][4]) {
+    constexpr int N2 = N * N, NF = N + 1;
+    const int q1 = ydir ? 2 : 1, q2 = ydir ? 1 : 2;
+    Real uc[N][4];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const double* u = eU + lbase + k * lstride;
+        uc[k][0] = (Real)u[0]; uc[k][1] = (Real)u[q1 * N2]; uc[k][2] = (Real)u[q2 * N2]; uc[k][3] = (Real)u[3 * N2];
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[i][q] = Dsd[i * NF] * fm[q] + Dsd[i * NF + N] * fp[q];
+#pragma unroll
+    for (int m = 1; m < N; m++) {
+        Real uf[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            Real t = 0;
+#pragma unroll
+            for (int j = 0; j < N; j++) t += Isf[m * N + j] * uc[j][q];
+            uf[q] = t;
+        }
+        Real F[4];
+        ph.template flux<0>(ph.to_prim_sane(uf), F);
+#pragma unroll
+        for (int i = 0; i < N; i++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[i][q] += Dsd[i * NF + m] * F[q];
+    }
+}
+
 // Register budget: a line keeps its p+1 node states and 4(p+1) accumulators
 // live ((p+1)*9 values per lane, twice the registers in FP64).  The floor on
 // waves per SIMD lets FP64 at p >= 5 use the full 512-register file (VGPRs +
@@ -421,7 +471,10 @@ template <int P, int VOL, bool FULL> struct Occ<P, double, VOL, FULL> {
     // FP64 at p >= 6 (p = 5 with the Chandrashekar flux): the line state and
     // accumulators plus one pair's temporaries exceed 256 registers, so use
     // the whole file rather than spill
-    static constexpr int waves = (P >= 6 || (P == 5 && VOL == VOL_CHANDRASHEKAR)) ? 1 : 2;
+    // the same for the spectral-difference line (its p+1 conservative states
+    // and 4(p+1) accumulators) from p = 5 on
+    static constexpr int waves =
+        VOL == VOL_SD ? (P >= 5 ? 1 : 2) : (P >= 6 || (P == 5 && VOL == VOL_CHANDRASHEKAR)) ? 1 : 2;
 };
 
 // One tile (E consecutive elements) of one stage.  `lead` marks the one tile
@@ -438,6 +491,8 @@ template <int P, typename Real, bool FULL, int VOL, bool PIN = (sizeof(Real) == 
           bool U0_EARLY = !FULL>
 __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int tile, bool lead) {
     static_assert(UP_LDS || !FULL, "the limiter needs the stage values in LDS");
+    constexpr bool SD = VOL == VOL_SD;
+    static_assert(UP_LDS || !SD, "the SD line reads the stage input from LDS");
     constexpr int N = P + 1, N2 = N * N;
     using Mx = M<Real>;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -477,7 +532,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     double* sTrho = sEl + 4 * E;                     // [E]
     unsigned long long* sTheta = (unsigned long long*)(sEl + 5 * E);  // [E]
 
-    const bool need_lam = FULL && (a.av || (a.mode == MODE_RHS && a.lam_el != nullptr));
+    // SD: no AV; its lean kernel gathers the element wave speed for lam_el
+    const bool need_lam = (FULL || SD) && (a.av || (a.mode == MODE_RHS && a.lam_el != nullptr));
 
     // line owned by this thread in phase B: element le, x-line (u, v as
     // stored) or y-line (u <-> v swapped); node k of the line is lbase + k*lstride
@@ -626,7 +682,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         if (!go) return;
     }
 
-    if (FULL && (need_lam || a.av)) {
+    if ((FULL || SD) && (need_lam || a.av)) {
         // element maxima gathered with LDS atomics on the ordered bit patterns
         // of non-negative doubles (order-independent, so deterministic)
         if (tid < E) { sLamE[tid] = 0ull; sThE[tid] = 0ull; }
@@ -639,7 +695,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         const int e = s / N2, nd = s - e * N2;
         double W[4] = {1.0, 0.0, 0.0, 1.0}, ri = 1.0;
         if (s < nE * N2) {
-            if (UP_LDS && staged) {
+            if (UP_LDS && (staged || SD)) {
 #pragma unroll
                 for (int q = 0; q < 4; q++) sU[e * 4 * N2 + q * N2 + nd] = Ul[j][q];
             }
@@ -649,12 +705,14 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         Real* p = sW + s * 5;
         p[0] = (Real)0.5 * w.r; p[1] = (Real)0.5 * w.u; p[2] = (Real)0.5 * w.v;
         p[3] = (Real)0.5 * w.p; p[4] = (Real)0.5 * w.z;
-        if (FULL && need_lam) elem_max_lds<N2>(&sLamE[e], dbits(Cons64::wave_speed(W, g, ri)), s < nE * N2);
+        if ((FULL || SD) && need_lam)
+            elem_max_lds<N2>(&sLamE[e], dbits(SD ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri)),
+                             s < nE * N2);
     }
     __syncthreads();
 
     // ── AV pre-pass: velocity divergence, sensor, element viscosity (:483-521) ──
-    if (FULL && a.av) {
+    if (FULL && !SD && a.av) {
         if (is_line) {
             Real un[N];
 #pragma unroll
@@ -737,7 +795,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                 }
                 // split forms: F*(face) (the diagonal volume term cancels the
                 // -f(own) of the strong-form surface term exactly on LGL
-                // nodes); collocation: F*(face) - f(own)
+                // nodes); collocation: F*(face) - f(own); SD: F*(face), the
+                // flux on the end flux node
                 constexpr bool JUMP = VOL == VOL_COLLOCATION;
                 Real dj[4];
                 if (a.surf == SURF_RUSANOV) {
@@ -752,7 +811,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                         for (int q = 0; q < 4; q++) dj[q] -= fo[q];
                     }
                 }
-                const Real c = (f ? -jac : jac) * winv[f ? N - 1 : 0];
+                const Real c = SD ? (Real)1 : (f ? -jac : jac) * winv[f ? N - 1 : 0];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     Real v = c * dj[q];
@@ -761,11 +820,17 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             }
         }
 
-        line_volume<N, Real, VOL, PIN>(ph, hw, Dv, cm, cp, acc);
+        if (SD) {
+            const Real* Isf = D + 3 * N * N + 2 * N;     // [N+1][N]
+            const Real* Dsd = Isf + (N + 1) * N;         // [N][N+1]
+            sd_line<N, Real>(ph, sU + le * 4 * N2, lbase, lstride, ydir, Isf, Dsd, cm, cp, acc);
+        } else {
+            line_volume<N, Real, VOL, PIN>(ph, hw, Dv, cm, cp, acc);
+        }
 
         // ── artificial viscosity along the line: -jac W^-1 D^T W (nu jac D Uc)
         //    = -nu jac^2 Lav Uc, Lav = W^-1 D^T W D precomputed (one pass) ──
-        if (FULL && a.av) {
+        if (FULL && !SD && a.av) {
             const Real f = -sNu[le] * jac * jac;
             const Real* Lav = D + 2 * N * N + 2 * N;
             Real uc[N][4];      // conserved variables of the line's nodes (u, v as rotated)
@@ -831,12 +896,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
 #pragma unroll
             for (int q = 0; q < 4; q++) a.out[q * S + o] = (double)r[q * N2];
         }
-        if (FULL && a.lam_el != nullptr && tid < nE) a.lam_el[el0 + tid] = bitsd(sLamE[tid]);
+        if ((FULL || SD) && a.lam_el != nullptr && tid < nE) a.lam_el[el0 + tid] = bitsd(sLamE[tid]);
         return;
     }
 
     // MODE_STAGE: U_s = ca*U0 + cb*(Up + dt*R), sanitize, [limit], write, [lambda]
-    const bool lim = FULL && a.limiter;
+    const bool lim = FULL && !SD && a.limiter;     // Zhang-Shu; SD repairs per node below
     const bool want_lambda = a.stage == 3;
     double lmax = 0.0;
     gptr outq[4];
@@ -861,6 +926,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             for (int q = 0; q < 4; q++) Us[q] = Ui[q] + dt * (double)r[q * N2];
         }
         Cons64::sanitize_cons(Us, gs);
+        if (SD && a.limiter) Cons64::repair(Us, gs, a.eps_rho, a.eps_p);
         if (lim) {
 #pragma unroll
             for (int q = 0; q < 4; q++) up[q * N2] = Us[q];
@@ -870,7 +936,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             if (want_lambda) {
                 double W[4], ri;
                 Cons64::to_prim(Us, W, gs, ri);
-                lmax = nanmax(lmax, Cons64::wave_speed(W, g, ri));
+                lmax = nanmax(lmax, SD ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri));
             }
         }
     }

@@ -20,6 +20,7 @@ struct dg_ctx {
     int N, N2, NEL;
     double h;
     double x[16], w[16], D[256];        // host FP64 operator tables
+    double xf[17], Isf[17 * 16], Dfs[16 * 17];   // spectral-difference tables (NF = N + 1)
     void* d_tab64 = nullptr;             // D, w, winv in FP64
     void* d_tab32 = nullptr;             // D, w, winv in FP32
     double* d_U1 = nullptr;              // stage buffers of the fused stepping
@@ -128,11 +129,71 @@ static void diff_matrix(int n, const double* x, double* D) {
     }
 }
 
+// ── spectral-difference operators (fr_lobatto_2d.cu:76-166, :840-866) ──────
+// Flux nodes {-1, Gauss-Legendre nodes, +1}: the reference's literal GL
+// nodes for p <= 3 (:84-89), else the roots of L_p by Newton in long double
+// from Chebyshev guesses.  Isf[k][j] = l_j(xf_k) (solution-node Lagrange
+// basis at the flux nodes, :99-118).  Dfs[i][k] = l^f_k'(x_i), the flux-node
+// basis differentiated at the solution nodes (what :17-24 defines), written
+// as sum_{m != k} 1/(xf_k - xf_m) prod_{l != k,m} (x_i - xf_l)/(xf_k - xf_l),
+// which has no 1/(x_i - xf_m) term: the end solution nodes coincide with
+// flux nodes, where the reference's construction (:120-166) divides by zero.
+static void sd_flux_nodes(int p, double* xf) {
+    static const double gl[3][3] = {{0.0}, {-0.5773502691896258, 0.5773502691896258},
+                                    {-0.7745966692414834, 0.0, 0.7745966692414834}};
+    xf[0] = -1.0;
+    xf[p + 1] = 1.0;
+    if (p <= 3) {
+        for (int j = 0; j < p; j++) xf[1 + j] = gl[p - 1][j];
+        return;
+    }
+    for (int j = 0; j < p; j++) {
+        long double t = -cosl(3.14159265358979323846264338327950288L * (j + 0.75L) / (p + 0.5L));
+        for (int it = 0; it < 100; it++) {
+            long double L, dL;
+            legendre_pair(p, t, &L, &dL);
+            long double step = L / dL;
+            t -= step;
+            if (fabsl(step) < 1e-19L) break;
+        }
+        xf[1 + j] = (double)t;
+    }
+    for (int j = 0; j < p / 2; j++) {   // exact mirror symmetry
+        double a = 0.5 * (xf[p - j] - xf[1 + j]);
+        xf[1 + j] = -a; xf[p - j] = a;
+    }
+    if (p % 2 == 1) xf[1 + p / 2] = 0.0;
+}
+
+static void sd_tables(int N, const double* xs, const double* xf, double* Isf, double* Dfs) {
+    const int NF = N + 1;
+    for (int k = 0; k < NF; k++)
+        for (int j = 0; j < N; j++) {
+            long double v = 1;
+            for (int m = 0; m < N; m++)
+                if (m != j) v *= ((long double)xf[k] - xs[m]) / ((long double)xs[j] - xs[m]);
+            Isf[k * N + j] = (double)v;
+        }
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < NF; k++) {
+            long double d = 0;
+            for (int m = 0; m < NF; m++) {
+                if (m == k) continue;
+                long double t = 1 / ((long double)xf[k] - xf[m]);
+                for (int l = 0; l < NF; l++)
+                    if (l != k && l != m) t *= ((long double)xs[i] - xf[l]) / ((long double)xf[k] - xf[l]);
+                d += t;
+            }
+            Dfs[i * NF + k] = (double)d;
+        }
+}
+
 // ── small kernels ──────────────────────────────────────────────────────────
 struct ICArgs {
     int kind, K, N;
     double x0, y0, L, h, gamma;
     double rho_hi, p_hi, eps, xc, yc, u0, v0;
+    double sod_width;     // circular Sod front width in units of h
     unsigned long long seed;
     double amp;
     const double* xi;
@@ -198,7 +259,7 @@ __global__ void ic_kernel(double* Q, ICArgs a) {
         if (a.kind == DG_IC_CIRCULAR_SOD) {
             const double cx = a.x0 + 0.5 * a.L, cy = a.y0 + 0.5 * a.L;
             const double rr = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy));
-            const double phi = 0.5 * (1.0 + tanh((0.25 * a.L - rr) / (0.5 * a.h)));
+            const double phi = 0.5 * (1.0 + tanh((0.25 * a.L - rr) / (a.sod_width * a.h)));
             r = 0.125 + (a.rho_hi - 0.125) * phi; u = 0.0; v = 0.0; p = 0.1 + (a.p_hi - 0.1) * phi;
         } else if (a.kind == DG_IC_VORTEX) {
             vortex_state(x, y, a, 0, r, u, v, p);
@@ -236,13 +297,14 @@ __global__ void rk3_kernel(double* Uo, const double* U0, const double* Up, const
 
 // Max wave speed of a state, for the first step of the fused stepping
 // (into the partial slots `parts`, as the stage-3 kernels do).
-__global__ void lambda_kernel(const double* Q, size_t S, double g, unsigned long long* parts) {
+__global__ void lambda_kernel(const double* Q, size_t S, double g, int sd, unsigned long long* parts) {
     __shared__ double sred[BLOCK / 64];
     double m = 0.0;
     for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < S; n += (size_t)blockDim.x * gridDim.x) {
-        double W[4];
-        load_prim64(Q, S, n, g, W);
-        m = nanmax(m, Cons64::wave_speed(W, g));
+        double W[4], ri;
+        const double U[4] = {Q[n], Q[S + n], Q[2 * S + n], Q[3 * S + n]};
+        Cons64::to_prim(U, W, g, ri);
+        m = nanmax(m, sd ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri));
     }
     block_max_atomic(m, &parts[blockIdx.x & (LAM_PARTS - 1)], sred);
 }
@@ -282,6 +344,16 @@ __global__ void state_reset_kernel(DevState* st, double t0) {
 }
 
 __global__ void slot_reset_kernel(unsigned long long* slot) { *slot = 0ull; }
+
+// nodal positivity repair of the SD sibling, in place (fr_lobatto_2d.cu:589-615)
+__global__ void repair_kernel(double* Q, size_t S, double g, double eps_rho, double eps_p) {
+    const Gas64 gs{g, g - 1.0, 1e-14 / (g - 1.0)};
+    for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < S; n += (size_t)blockDim.x * gridDim.x) {
+        double U[4] = {Q[n], Q[S + n], Q[2 * S + n], Q[3 * S + n]};
+        Cons64::repair(U, gs, eps_rho, eps_p);
+        for (int q = 0; q < 4; q++) Q[q * S + n] = U[q];
+    }
+}
 
 // GLL-weighted integrals of rho, rho u, rho v, E and -rho s/(g-1); one
 // partial per block, summed on the host in block order (deterministic).
@@ -456,10 +528,12 @@ struct StageCall {
 static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd, hipStream_t stream) {
     const dg_config& f = c->cfg;
     const bool fp32 = f.precision == DG_MIXED;
+    const bool sd = f.scheme == DG_SCHEME_SD;
     const bool lam = s.mode == MODE_RHS && s.lam_el != nullptr;
-    const bool lim = s.mode == MODE_STAGE && f.limiter;
-    const bool full = f.av || lim || lam;
-    LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, s.mode == MODE_STAGE, f.av != 0, lam, lim);
+    const bool lim = s.mode == MODE_STAGE && f.limiter && !sd;   // Zhang-Shu scratch
+    const bool full = !sd && (f.av || lim || lam);
+    // the SD line reads the FP64 stage input from LDS in both modes
+    LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, s.mode == MODE_STAGE || sd, f.av != 0, lam, lim);
     KArgs a;
     memset(&a, 0, sizeof(a));
     a.Up = s.Up; a.U0 = s.U0; a.out = s.out; a.lam_el = s.lam_el;
@@ -467,7 +541,7 @@ static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd, hipStream
     a.w64 = (const double*)c->d_tab64 + c->N2;
     a.st = c->d_st;
     a.P = f.p; a.K = f.K; a.NEL = c->NEL; a.E = E; a.xcd = xcd;
-    a.mode = s.mode; a.stage = s.stage; a.vol = f.volume_flux; a.surf = f.surface_flux; a.bc = f.bc;
+    a.mode = s.mode; a.stage = s.stage; a.vol = sd ? VOL_SD : f.volume_flux; a.surf = f.surface_flux; a.bc = f.bc;
     a.av = f.av; a.limiter = f.limiter; a.check_blowup = s.check_blowup;
     a.slot_cur = s.slot_cur; a.slot_next = s.slot_next;
     a.h = c->h; a.gamma = f.gamma; a.ca = s.ca; a.cb = s.cb; a.cfl = s.cfl; a.t_end = s.t_end;
@@ -534,6 +608,7 @@ void dg_config_default(dg_config* f) {
     f->av = 1; f->limiter = 1; f->precision = DG_FP64;
     f->av_cav = 0.25; f->av_ksensor = 0.1; f->eps_rho = 1e-6; f->eps_p = 1e-6;
     f->elems_per_block = 0; f->xcd_remap = -1;
+    f->scheme = DG_SCHEME_DGSEM;
 }
 
 void dg_ic_default(dg_ic_params* ic, int kind) {
@@ -573,6 +648,16 @@ int dg_operators(const dg_ctx* c, double* x, double* w, double* D) {
     return DG_OK;
 }
 
+int dg_sd_operators(const dg_ctx* c, double* xf, double* Isf, double* Dfs) {
+    if (!c) return DG_ERR_INVALID_ARG;
+    if (c->cfg.scheme != DG_SCHEME_SD) return DG_ERR_INVALID_ARG;
+    const int NF = c->N + 1;
+    if (xf) memcpy(xf, c->xf, sizeof(double) * NF);
+    if (Isf) memcpy(Isf, c->Isf, sizeof(double) * NF * c->N);
+    if (Dfs) memcpy(Dfs, c->Dfs, sizeof(double) * c->N * NF);
+    return DG_OK;
+}
+
 int dg_destroy(dg_ctx* c) {
     if (!c) return DG_OK;
     hipFree(c->d_tab64); hipFree(c->d_tab32); hipFree(c->d_U1); hipFree(c->d_U2);
@@ -591,6 +676,7 @@ int dg_create(const dg_config* cfg, void* stream, dg_ctx** out) {
     if (cfg->K < 1 || cfg->p < 1 || cfg->p > PMAX_GPU || !(cfg->L > 0) || !(cfg->gamma > 1.0) ||
         cfg->volume_flux < 0 || cfg->volume_flux > 2 || cfg->surface_flux < 0 || cfg->surface_flux > 1 ||
         cfg->bc < 0 || cfg->bc > 1 || cfg->precision < 0 || cfg->precision > 1 ||
+        cfg->scheme < 0 || cfg->scheme > 1 || (cfg->scheme == DG_SCHEME_SD && cfg->av) ||
         (long long)cfg->K * cfg->K > (1LL << 30) / ((cfg->p + 1) * (cfg->p + 1)))
         return DG_ERR_INVALID_ARG;
     dg_ctx* c = new dg_ctx();
@@ -600,18 +686,24 @@ int dg_create(const dg_config* cfg, void* stream, dg_ctx** out) {
     c->h = cfg->L / cfg->K;
     lgl_nodes(cfg->p, c->x, c->w);
     diff_matrix(c->N, c->x, c->D);
+    sd_flux_nodes(cfg->p, c->xf);
+    sd_tables(c->N, c->x, c->xf, c->Isf, c->Dfs);
 
     const int Emax = max_elems_per_block(c->N);
     c->E = cfg->elems_per_block > 0 ? std::min(cfg->elems_per_block, Emax) : default_E(c->N, c->NEL);
     c->xcd = cfg->xcd_remap >= 0 ? (cfg->xcd_remap != 0) : 1;
 
-    // tables: D[N2], w[N], winv[N], Dv[N2], Lav[N2].  Dv is the volume matrix,
-    // D scaled by the volume term's factor -2/h (collocation) or -4/h (the
-    // split form's 2 D F#), so that the line loop needs no scaling pass; Lav =
-    // W^-1 D^T W D is the artificial-viscosity operator of one line (the
-    // reference's two passes, D then W^-1 D^T W, in one matrix)
-    std::vector<double> t64(3 * c->N2 + 2 * c->N);
-    std::vector<float> t32(3 * c->N2 + 2 * c->N);
+    // tables: D[N2], w[N], winv[N], Dv[N2], Lav[N2], Isf[(N+1)N], Dsd[N(N+1)].
+    // Dv is the volume matrix, D scaled by the volume term's factor -2/h
+    // (collocation) or -4/h (the split form's 2 D F#), so that the line loop
+    // needs no scaling pass; Lav = W^-1 D^T W D is the artificial-viscosity
+    // operator of one line (the reference's two passes, D then W^-1 D^T W, in
+    // one matrix); Isf and Dsd = -(2/h) Dfs are the spectral-difference
+    // sibling's interpolation and scaled flux-derivative matrices
+    const int NF = c->N + 1;
+    const size_t off_sd = 3 * c->N2 + 2 * c->N;
+    std::vector<double> t64(off_sd + 2 * NF * c->N);
+    std::vector<float> t32(t64.size());
     const double vscale = (cfg->volume_flux == DG_VOL_COLLOCATION ? -1.0 : -2.0) * (2.0 / c->h);
     for (int i = 0; i < c->N2; i++) t64[i] = c->D[i];
     for (int i = 0; i < c->N; i++) { t64[c->N2 + i] = c->w[i]; t64[c->N2 + c->N + i] = 1.0 / c->w[i]; }
@@ -622,6 +714,10 @@ int dg_create(const dg_config* cfg, void* stream, dg_ctx** out) {
             for (int k = 0; k < c->N; k++) sum += c->w[k] * c->D[k * c->N + i] * c->D[k * c->N + m];
             t64[2 * c->N2 + 2 * c->N + i * c->N + m] = sum / c->w[i];
         }
+    for (int i = 0; i < NF * c->N; i++) {
+        t64[off_sd + i] = c->Isf[i];
+        t64[off_sd + NF * c->N + i] = -(2.0 / c->h) * c->Dfs[i];
+    }
     for (size_t i = 0; i < t64.size(); i++) t32[i] = (float)t64[i];
 
     const size_t S = (size_t)c->NEL * c->N2;
@@ -657,6 +753,8 @@ static ICArgs ic_args(const dg_ctx* c, const dg_ic_params* ic) {
     a.x0 = c->cfg.x0; a.y0 = c->cfg.y0; a.L = c->cfg.L; a.h = c->h; a.gamma = c->cfg.gamma;
     a.rho_hi = ic->rho_hi; a.p_hi = ic->p_hi; a.eps = ic->vortex_eps; a.xc = ic->xc; a.yc = ic->yc;
     a.u0 = ic->u0; a.v0 = ic->v0; a.seed = ic->seed; a.amp = ic->noise_amp;
+    // front width: 0.5h (dgsem_lobatto_2d.cu:710), 2h (fr_lobatto_2d.cu:648)
+    a.sod_width = c->cfg.scheme == DG_SCHEME_SD ? 2.0 : 0.5;
     a.xi = c->d_xi;
     return a;
 }
@@ -668,12 +766,20 @@ int dg_init_state(dg_ctx* c, const dg_ic_params* ic, double* Q) {
     const size_t S = (size_t)c->NEL * c->N2;
     hipLaunchKernelGGL(ic_kernel, dim3(grid_for(S, BLOCK)), dim3(BLOCK), 0, c->stream, Q, a);
     HIPCK(c, hipGetLastError());
-    if (c->cfg.limiter) return dg_limit(c, Q);
+    // the SD driver applies no limiter to its IC (fr_lobatto_2d.cu:921-923)
+    if (c->cfg.limiter && c->cfg.scheme != DG_SCHEME_SD) return dg_limit(c, Q);
     return DG_OK;
 }
 
 int dg_limit(dg_ctx* c, double* Q) {
     if (!c || !Q) return fail(c, DG_ERR_INVALID_ARG, "dg_limit: null argument");
+    if (c->cfg.scheme == DG_SCHEME_SD) {
+        const size_t S = (size_t)c->NEL * c->N2;
+        hipLaunchKernelGGL(repair_kernel, dim3(grid_for(S, BLOCK)), dim3(BLOCK), 0, c->stream, Q, S, c->cfg.gamma,
+                           c->cfg.eps_rho, c->cfg.eps_p);
+        HIPCK(c, hipGetLastError());
+        return DG_OK;
+    }
     const int E = std::max(1, max_elems_per_block(c->N) / 2);
     HIPCK(c, limit_launcher(c->cfg.p)(Q, (const double*)c->d_tab64 + c->N2, c->NEL, E, c->cfg.gamma,
                                        c->cfg.eps_rho, c->cfg.eps_p, c->stream));
@@ -736,7 +842,7 @@ static int advance_impl(dg_ctx* c, double* Q, int nsteps, double cfl, double t_e
     const double t0 = t_inout ? *t_inout : 0.0;
     hipLaunchKernelGGL(state_reset_kernel, dim3(1), dim3(1), 0, c->stream, c->d_st, t0);
     hipLaunchKernelGGL(lambda_kernel, dim3(grid_for(S, BLOCK)), dim3(BLOCK), 0, c->stream, Q, S, c->cfg.gamma,
-                       c->d_st->lam_part[0]);
+                       (int)(c->cfg.scheme == DG_SCHEME_SD), c->d_st->lam_part[0]);
     HIPCK(c, hipGetLastError());
     int rc = DG_OK, n = 0;
     if (nsteps > 0 && (rc = advance_step(c, Q, n++, cfl, t_end, E, xcd, c->stream))) return rc;
@@ -1078,7 +1184,7 @@ int dg_batch_create(const dg_config* cfgs, int n, void* stream, dg_batch** out) 
     for (int i = 0; i < n; i++) {
         const dg_config& f = cfgs[i];
         if (f.volume_flux != c0.volume_flux || f.surface_flux != c0.surface_flux || f.bc != c0.bc ||
-            f.precision != c0.precision || f.av || f.limiter)
+            f.precision != c0.precision || f.av || f.limiter || f.scheme != DG_SCHEME_DGSEM)
             return DG_ERR_INVALID_ARG;
     }
     dg_batch* b = new dg_batch();

@@ -122,6 +122,19 @@ struct Cons64 {
         double c2 = g * W[3] * (W[0] >= 1e-12 ? ri : rcp(1e-12));
         return fabs(W[1]) + fabs(W[2]) + sqrt(fmax(c2, 1e-14));
     }
+    // the spectral-difference sibling's element wave speed: |u|+|v| plus
+    // sound_speed(p, rho) of the sanitized prims, without the 1e-12 density
+    // floor (fr_lobatto_2d.cu:168-172, :515-518); ri = 1/W[0] from to_prim
+    static __device__ __forceinline__ double wave_speed_sd(const double W[4], double g, double ri) {
+        return fabs(W[1]) + fabs(W[2]) + sqrt(fmax(g * W[3] * ri, 1e-14));
+    }
+    // nodal positivity repair of the SD sibling (fr_lobatto_2d.cu:589-615):
+    // rho < eps_rho -> rho = eps_rho, zero momentum; then p < eps_p -> E reset
+    static __device__ __forceinline__ void repair(double U[4], const Gas64& gs, double eps_rho, double eps_p) {
+        if (U[0] < eps_rho) { U[0] = eps_rho; U[1] = 0.0; U[2] = 0.0; }
+        const double m2 = 0.5 * (U[1] * U[1] + U[2] * U[2]);
+        if (gs.gm1 * (U[3] - m2 / fmax(U[0], 1e-14)) < eps_p) U[3] = eps_p / gs.gm1 + m2 / U[0];
+    }
 };
 
 template <typename Real>
@@ -141,6 +154,28 @@ template <typename Real> struct Phys {
     __device__ __forceinline__ Real sound_speed(Real p, Real rho) const {   // :184-188
         Real cs2 = Mx::div(g * p, rho);
         return Mx::sqrt_(cs2 > (Real)1e-14 ? cs2 : (Real)1e-14);
+    }
+    // cons_to_prim_sane in the compute precision (:223-229): the state the
+    // spectral-difference line interpolates to a flux node (z is not set;
+    // only the physical flux uses the result)
+    __device__ __forceinline__ Prim<Real> to_prim_sane(const Real U[4]) const {
+        Prim<Real> w;
+        w.r = Mx::fmax_(U[0], (Real)1e-14);
+        const Real ri = Mx::rcp(w.r);
+        const Real ke = (Real)0.5 * (U[1] * U[1] + U[2] * U[2]) * ri;
+        Real E = U[3];
+        if (gm1 * (E - ke) < (Real)1e-14) E = (Real)1e-14 * inv_gm1 + ke;
+        w.u = U[1] * ri;
+        w.v = U[2] * ri;
+        w.p = Mx::fmax_(gm1 * (E - ke), (Real)1e-14);
+        w.z = (Real)0;
+        if (w.r < (Real)1e-5) {
+            w.u = (Real)0; w.v = (Real)0;
+        } else {
+            w.u = Mx::fmax_(Mx::fmin_(w.u, (Real)1e4), (Real)-1e4);
+            w.v = Mx::fmax_(Mx::fmin_(w.v, (Real)1e4), (Real)-1e4);
+        }
+        return w;
     }
     // physical flux along x (dir 0) or y (dir 1), :232-241
     template <int DIR>

@@ -37,6 +37,7 @@ SIGNATURES = {
     "dg_state_size": (C.c_longlong, [_vp]),
     "dg_num_elements": (C.c_longlong, [_vp]),
     "dg_operators": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "dg_sd_operators": (C.c_int, [_vp, _dp, _dp, _dp]),
     "dg_init_state": (C.c_int, [_vp, C.POINTER(DGICParams), _vp]),
     "dg_compute_rhs": (C.c_int, [_vp, _vp, _vp, _vp]),
     "dg_rk3_stage": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_double]),
@@ -142,6 +143,16 @@ class Solver:
         self._check(self.lib.dg_operators(self._ctx, x.ctypes.data_as(_dp), w.ctypes.data_as(_dp),
                                           D.ctypes.data_as(_dp)))
         return x, w, D.reshape(self.NN, self.NN)
+
+    def sd_operators(self):
+        """Spectral-difference tables of an SD context: flux nodes xf[p+2],
+        Isf[p+2, p+1] and Dfs[p+1, p+2] (fr_lobatto_2d.cu:99-166)."""
+        import numpy as np
+        nf = self.NN + 1
+        xf = np.zeros(nf); Isf = np.zeros(nf * self.NN); Dfs = np.zeros(self.NN * nf)
+        self._check(self.lib.dg_sd_operators(self._ctx, xf.ctypes.data_as(_dp), Isf.ctypes.data_as(_dp),
+                                             Dfs.ctypes.data_as(_dp)))
+        return xf, Isf.reshape(nf, self.NN), Dfs.reshape(self.NN, nf)
 
     # ── reference operations ──
     def init_state(self, ic, Q=None):

@@ -50,6 +50,20 @@ enum dg_precision {
     DG_FP64  = 0,
     DG_MIXED = 1               /* FP32 volume/surface, FP64 state and reductions */
 };
+/* Spatial scheme.  DG_SCHEME_SD is the sibling solver of fr_lobatto_2d.cu:
+ * spectral difference on the same LGL solution nodes and state layout, with
+ * p + 2 flux nodes per direction {-1, Gauss-Legendre nodes, +1}; the
+ * conservative state is interpolated to the interior flux nodes, the
+ * physical flux is evaluated there, the Riemann flux (surface_flux) on the
+ * two end nodes, and the flux polynomial is differentiated at the solution
+ * nodes (compute_sd_rhs, fr_lobatto_2d.cu:332-525).  It has no volume flux
+ * and no artificial viscosity (av must be 0); its `limiter` is the
+ * reference's nodal positivity repair (fr_lobatto_2d.cu:589-615) instead of
+ * the Zhang-Shu limiter.  Not available in batched mode. */
+enum dg_scheme {
+    DG_SCHEME_DGSEM = 0,       /* dgsem_lobatto_2d.cu                            */
+    DG_SCHEME_SD    = 1        /* fr_lobatto_2d.cu                               */
+};
 enum dg_ic_kind {
     DG_IC_CIRCULAR_SOD      = 0,  /* ic_kernel, :692-717                          */
     DG_IC_VORTEX            = 1,  /* isentropic vortex, q1_dg_euler_2d.cu:2166-2188 */
@@ -76,6 +90,7 @@ typedef struct dg_config {
     double eps_p;
     int    elems_per_block; /* tuning: elements per workgroup, 0 = heuristic     */
     int    xcd_remap;       /* tuning: XCD-aware block->tile map, -1 = default on */
+    int    scheme;          /* enum dg_scheme (DG_SCHEME_DGSEM by default)      */
 } dg_config;
 
 /* Initial-condition parameters (unused fields are ignored). */
@@ -110,13 +125,25 @@ long long   dg_num_elements(const dg_ctx* ctx);
 /* Copy the host-side operator tables: x[NN], w[NN], D[NN*NN] (row-major D_ik). */
 int         dg_operators(const dg_ctx* ctx, double* x, double* w, double* D);
 
+/* DG_SCHEME_SD only: the flux nodes xf[p+2], the interpolation matrix
+ * Isf[(p+2)*(p+1)] (row-major, Isf[k][j] = l_j(xf_k), the Lagrange basis of
+ * the solution nodes at the flux nodes) and the derivative matrix
+ * Dfs[(p+1)*(p+2)] (row-major, Dfs[i][k] = derivative of the flux nodes'
+ * Lagrange basis l^f_k at solution node x_i).  Replaces build_Isf /
+ * build_Dfs + cudaMemcpyToSymbol, fr_lobatto_2d.cu:99-166,840-866.  Any
+ * pointer may be NULL. */
+int         dg_sd_operators(const dg_ctx* ctx, double* xf, double* Isf, double* Dfs);
+
 /* Evaluate the IC at the LGL nodes into device state Q, then apply the
  * positivity limiter if it is enabled (ic_kernel + positivity_limiter,
- * dgsem_lobatto_2d.cu:692-717,1006-1007). */
+ * dgsem_lobatto_2d.cu:692-717,1006-1007).  DG_SCHEME_SD: the circular Sod
+ * front has the width 2h of fr_lobatto_2d.cu:633-657 (0.5h for DGSEM) and no
+ * limiter pass follows (fr_lobatto_2d.cu:921-923). */
 int         dg_init_state(dg_ctx* ctx, const dg_ic_params* ic, double* Q);
 
 /* du/dt of device state Q into RHS (FP64, same layout) and the per-element
- * max wave speed lam_el[NEL] (may be NULL).  compute_dg_rhs, :365-530. */
+ * max wave speed lam_el[NEL] (may be NULL).  compute_dg_rhs, :365-530;
+ * DG_SCHEME_SD: compute_sd_rhs, fr_lobatto_2d.cu:332-525. */
 int         dg_compute_rhs(dg_ctx* ctx, const double* Q, double* RHS, double* lam_el);
 
 /* One SSP-RK3 Shu-Osher stage followed by sanitize_cons and (if enabled) the
@@ -125,11 +152,15 @@ int         dg_compute_rhs(dg_ctx* ctx, const double* Q, double* RHS, double* la
  *   stage 2: Uout = 3/4*U0 + 1/4*(Uprev + dt*L)
  *   stage 3: Uout = 1/3*U0 + 2/3*(Uprev + dt*L)
  * Uprev is ignored for stage 1.  Uout may alias U0 (stage 3 of the reference
- * writes Q while Q0 holds the copy). */
+ * writes Q while Q0 holds the copy).  DG_SCHEME_SD: rk3_fr_s1/s2/s3 +
+ * positivity_limiter, fr_lobatto_2d.cu:542-615 (the same stages, the nodal
+ * repair). */
 int         dg_rk3_stage(dg_ctx* ctx, int stage, double* Uout, const double* U0,
                          const double* Uprev, const double* L, double dt);
 
-/* Zhang-Shu positivity limiter in place (positivity_limiter, :584-643). */
+/* Zhang-Shu positivity limiter in place (positivity_limiter, :584-643);
+ * DG_SCHEME_SD: the nodal repair (rho < eps_rho: rho = eps_rho and zero
+ * momentum; then p < eps_p: E reset to p = eps_p), fr_lobatto_2d.cu:589-615. */
 int         dg_limit(dg_ctx* ctx, double* Q);
 
 /* Max of lam_el[NEL] (device) -> *lam_max (host) and the CFL step

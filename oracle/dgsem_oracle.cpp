@@ -12,6 +12,8 @@
  * switches, absent from the reference and pinned only by closed forms
  * (see the tests): the Ranocha volume flux, the Rusanov surface flux, periodic
  * boundaries, LGL data for p > 3 and the vortex / Kelvin-Helmholtz ICs.
+ * The spectral-difference sibling (DG_SCHEME_SD) restates fr_lobatto_2d.cu
+ * the same way, except for its derivative matrix's end rows (see build_sd).
  *
  * Parity pins (tests/test_oracle_*.py):
  *   lgl / build_D .......... reference tables (:145-157), run_fsbp.py literals,
@@ -26,6 +28,12 @@
  *   limiter ................ mean preservation, identity on admissible data
  *   sensor / AV ............ parity unpinned beyond free-stream (no reference
  *                            vectors exist for them; see the oracle tests)
+ *   SD sibling ............. (fr_lobatto_2d.cu, tests/test_oracle_sd.py) flux
+ *                            nodes = Gauss-Legendre roots and the reference's
+ *                            literals, Isf / Dfs exact on polynomials of degree
+ *                            p / p+1, w^T Dfs telescopes, free stream,
+ *                            discrete conservation, closed wall box, order
+ *                            p+1 on the vortex, the nodal repair, the IC width
  *
  * Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp -shared).
  */
@@ -54,6 +62,9 @@ struct Ctx {
     double h;
     double xi[PMAX + 1], w[PMAX + 1], winv[PMAX + 1];
     double D[PMAX + 1][PMAX + 1];
+    /* spectral-difference sibling (fr_lobatto_2d.cu): NF = p + 2 flux nodes */
+    int NF;
+    double xf[PMAX + 2], Isf[PMAX + 2][PMAX + 1], Dfs[PMAX + 1][PMAX + 2];
     std::vector<double> Q0, Q1, Q2, RHS, lam;
 };
 
@@ -144,6 +155,97 @@ static void build_D(int n, const double* x, double D[PMAX + 1][PMAX + 1])
             }
         }
         D[i][i] = diag;
+    }
+}
+
+/* ── SD operators (fr_lobatto_2d.cu:76-166, :840-866) ────────────────────
+ * Flux nodes xf = {-1, Gauss-Legendre nodes, +1}, p + 2 of them: the
+ * reference's literal GL nodes for p <= 3 (:84-89); for p > 3 the roots of
+ * L_p, Newton from Chebyshev guesses in long double.
+ * Isf[k][j] = l_j(xf_k), the Lagrange basis of the solution nodes at the
+ * flux nodes (:99-118).
+ * Dfs[i][k] = l^f_k'(xs_i), the derivative of the flux nodes' Lagrange basis
+ * at the solution nodes, the quantity the reference defines (:17-24,
+ * :118-119).  Its construction (:120-166) is followed where xs_i is no flux
+ * node: l^f_k(xs_i) * sum_{m != k} 1/(xs_i - xf_m).  It is not followed
+ * where a solution node coincides with a flux node (the end nodes): there
+ * the reference multiplies l^f_k(xs_i) = 0 by a sum holding 1/0 (NaN) and
+ * gives the coincident entry the sign opposite to the derivative.  Here:
+ * xs_i = xf_k: sum_{m != k} 1/(xf_k - xf_m); xs_i = xf_m, m != k:
+ * prod_{l != k,m} (xf_m - xf_l) / prod_{l != k} (xf_k - xf_l).
+ */
+static const double ref_gl_interior[3][3] = {   /* fr_lobatto_2d.cu:84-89 */
+    {0.0},
+    {-0.5773502691896258, 0.5773502691896258},
+    {-0.7745966692414834, 0.0, 0.7745966692414834},
+};
+
+static void gauss_legendre(int n, double* x)
+{
+    if (n >= 1 && n <= 3) {
+        for (int i = 0; i < n; i++) x[i] = ref_gl_interior[n - 1][i];
+        return;
+    }
+    const long double PI = 3.14159265358979323846264338327950288L;
+    for (int j = 0; j < n; j++) {
+        long double t = -cosl(PI * (j + 0.75L) / (n + 0.5L));
+        for (int it = 0; it < 100; it++) {
+            long double L, dL;
+            legendre(n, t, &L, &dL);
+            long double dt = L / dL;
+            t -= dt;
+            if (fabsl(dt) < 1e-19L) break;
+        }
+        x[j] = (double)t;
+    }
+    for (int j = 0; j < n / 2; j++) {   /* exact symmetry */
+        double a = 0.5 * (x[n - 1 - j] - x[j]);
+        x[j] = -a; x[n - 1 - j] = a;
+    }
+    if (n % 2 == 1) x[n / 2] = 0.0;
+}
+
+static double lagrange_val(int n, const double* xs, int j, double x)   /* :99-107 */
+{
+    double v = 1.0;
+    for (int k = 0; k < n; k++)
+        if (k != j) v *= (x - xs[k]) / (xs[j] - xs[k]);
+    return v;
+}
+
+static void build_sd(Ctx& c)
+{
+    const int ns = c.NN, nf = c.NN + 1;
+    c.NF = nf;
+    c.xf[0] = -1.0;
+    gauss_legendre(c.p, c.xf + 1);
+    c.xf[nf - 1] = 1.0;
+    for (int k = 0; k < nf; k++)                                     /* :109-118 */
+        for (int j = 0; j < ns; j++) c.Isf[k][j] = lagrange_val(ns, c.xi, j, c.xf[k]);
+    for (int i = 0; i < ns; i++) {
+        int hit = -1;   /* flux node that coincides with xs_i */
+        for (int m = 0; m < nf; m++)
+            if (fabs(c.xi[i] - c.xf[m]) < 1e-14) hit = m;
+        for (int k = 0; k < nf; k++) {
+            double d = 0.0;
+            if (hit < 0) {
+                for (int m = 0; m < nf; m++)
+                    if (m != k) d += 1.0 / (c.xi[i] - c.xf[m]);
+                d *= lagrange_val(nf, c.xf, k, c.xi[i]);
+            } else if (hit == k) {
+                for (int m = 0; m < nf; m++)
+                    if (m != k) d += 1.0 / (c.xf[k] - c.xf[m]);
+            } else {
+                double num = 1.0, den = 1.0;
+                for (int l = 0; l < nf; l++) {
+                    if (l == k) continue;
+                    den *= c.xf[k] - c.xf[l];
+                    if (l != hit) num *= c.xf[hit] - c.xf[l];
+                }
+                d = num / den;
+            }
+            c.Dfs[i][k] = d;
+        }
     }
 }
 
@@ -481,10 +583,117 @@ static void rhs_element(const Ctx& c, const double* Q, double* RHS, double* lam,
     if (lam) lam[el] = lam_e;
 }
 
+/* ── compute_sd_rhs, fr_lobatto_2d.cu:332-525, one element ──────────────
+ * Per grid line: the conservative state interpolated to the interior flux
+ * nodes, converted (cons_to_prim_sane) and its physical flux evaluated; the
+ * Riemann flux on the two end flux nodes between the element's own
+ * sanitized end-node prims and the neighbour's; dF/dxi at the solution nodes
+ * through Dfs, subtracted with the factor 2/h.  x-lines, then y-lines.  The
+ * element wave speed is max |u| + |v| + c over the sanitized prims. */
+static void sd_rhs_element(const Ctx& c, const double* Q, double* RHS, double* lam, int el)
+{
+    const Phys ph{c.cfg.gamma};
+    const int N = c.K, nn = c.NN, nf = c.NF, nn2 = c.NN2;
+    const double jac = 2.0 / c.h;
+    const int ej = el / N, ei = el % N;
+    const bool periodic = (c.cfg.bc == DG_BC_PERIODIC);
+
+    std::vector<double> Qc(4 * nn2), Qe(4 * nn2), R(4 * nn2, 0.0);   /* :349-365 */
+    for (int n = 0; n < nn2; n++) {
+        double U[4], W[4];
+        for (int q = 0; q < 4; q++) U[q] = Q[sidx(c, q, el, n / nn, n % nn)];
+        ph.cons_to_prim_sane(U, W);
+        for (int q = 0; q < 4; q++) { Qc[q * nn2 + n] = U[q]; Qe[q * nn2 + n] = W[q]; }
+    }
+
+    for (int dir = 0; dir < 2; dir++) {
+        const double nx = dir == 0 ? 1.0 : 0.0, ny = dir == 0 ? 0.0 : 1.0;
+        for (int line = 0; line < nn; line++) {
+            /* node k of the line: x-lines (:367-438) run along in, y-lines (:440-511) along jn */
+            auto node = [&](int k) { return dir == 0 ? line * nn + k : k * nn + line; };
+            std::vector<double> Ff(4 * nf);
+            for (int k = 1; k < nf - 1; k++) {                   /* :370-390 */
+                double Uk[4], Wk[4], F4[4];
+                for (int q = 0; q < 4; q++) {
+                    double v = 0.0;
+                    for (int j = 0; j < nn; j++) v += c.Isf[k][j] * Qc[q * nn2 + node(j)];
+                    Uk[q] = v;
+                }
+                ph.cons_to_prim_sane(Uk, Wk);
+                ph.euler_flux_n(Wk[0], Wk[1], Wk[2], Wk[3], nx, ny, F4);
+                for (int q = 0; q < 4; q++) Ff[q * nf + k] = F4[q];
+            }
+            double Wm[4], Wp[4];                                    /* :392-397 */
+            for (int q = 0; q < 4; q++) { Wm[q] = Qe[q * nn2 + node(0)]; Wp[q] = Qe[q * nn2 + node(nn - 1)]; }
+            Phys::sanitize_prim(Wm); Phys::sanitize_prim(Wp);
+            const int pos = dir == 0 ? ei : ej;
+            {   /* low face (:399-413, :472-486) */
+                double Wn[4], F[4];
+                if (pos > 0 || periodic) {
+                    int pn = pos > 0 ? pos - 1 : N - 1;
+                    if (dir == 0) nb_prim(c, ph, Q, ej * N + pn, line, nn - 1, Wn);
+                    else nb_prim(c, ph, Q, pn * N + ei, nn - 1, line, Wn);
+                } else {
+                    Phys::wall_state(Wm, dir, Wn);
+                }
+                surface_flux(c, ph, Wn, Wm, nx, ny, F);
+                for (int q = 0; q < 4; q++) Ff[q * nf + 0] = F[q];
+            }
+            {   /* high face (:414-428, :487-501) */
+                double Wn[4], F[4];
+                if (pos < N - 1 || periodic) {
+                    int pn = pos < N - 1 ? pos + 1 : 0;
+                    if (dir == 0) nb_prim(c, ph, Q, ej * N + pn, line, 0, Wn);
+                    else nb_prim(c, ph, Q, pn * N + ei, 0, line, Wn);
+                } else {
+                    Phys::wall_state(Wp, dir, Wn);
+                }
+                surface_flux(c, ph, Wp, Wn, nx, ny, F);
+                for (int q = 0; q < 4; q++) Ff[q * nf + nf - 1] = F[q];
+            }
+            for (int i = 0; i < nn; i++)                           /* :430-437 */
+                for (int q = 0; q < 4; q++) {
+                    double dF = 0.0;
+                    for (int k = 0; k < nf; k++) dF += c.Dfs[i][k] * Ff[q * nf + k];
+                    R[q * nn2 + node(i)] -= jac * dF;
+                }
+        }
+    }
+
+    double lam_el = 0.0;                                            /* :513-522 */
+    for (int n = 0; n < nn2; n++) {
+        double rho = Qe[n], u = Qe[nn2 + n], v = Qe[2 * nn2 + n], p = Qe[3 * nn2 + n];
+        lam_el = std::max(lam_el, fabs(u) + fabs(v) + ph.sound_speed(p, rho));
+        for (int q = 0; q < 4; q++) RHS[sidx(c, q, el, n / nn, n % nn)] = R[q * nn2 + n];
+    }
+    if (lam) lam[el] = lam_el;
+}
+
 static void compute_rhs(const Ctx& c, const double* Q, double* RHS, double* lam)
 {
+    const bool sd = c.cfg.scheme == DG_SCHEME_SD;
 #pragma omp parallel for schedule(static)
-    for (int el = 0; el < c.NEL; el++) rhs_element(c, Q, RHS, lam, el);
+    for (int el = 0; el < c.NEL; el++) {
+        if (sd) sd_rhs_element(c, Q, RHS, lam, el);
+        else rhs_element(c, Q, RHS, lam, el);
+    }
+}
+
+/* ── nodal positivity repair of the SD sibling, fr_lobatto_2d.cu:589-615 ── */
+static void sd_repair(const Ctx& c, double* Q)
+{
+    const Phys ph{c.cfg.gamma};
+    const double eps_rho = c.cfg.eps_rho, eps_p = c.cfg.eps_p;
+    const size_t S = (size_t)c.NEL * c.NN2;
+#pragma omp parallel for schedule(static)
+    for (long long n = 0; n < (long long)S; n++) {
+        double U[4];
+        for (int q = 0; q < 4; q++) U[q] = Q[q * S + n];
+        if (U[0] < eps_rho) { U[0] = eps_rho; U[1] = 0.0; U[2] = 0.0; }
+        double p = ph.pressure_from_cons(U);
+        if (p < eps_p) U[3] = eps_p / (ph.g - 1.0) + 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0];
+        for (int q = 0; q < 4; q++) Q[q * S + n] = U[q];
+    }
 }
 
 /* ── SSP-RK3 stages, :533-576 ─────────────────────────────────────────── */
@@ -508,7 +717,7 @@ static void rk3_stage(const Ctx& c, int s, double* Uout, const double* U0, const
 }
 
 /* ── Zhang-Shu positivity limiter, :584-643 ───────────────────────────── */
-static void limiter(const Ctx& c, double* Q)
+static void zs_limiter(const Ctx& c, double* Q)
 {
     const Phys ph{c.cfg.gamma};
     const int nn = c.NN, nn2 = c.NN2;
@@ -557,6 +766,13 @@ static void limiter(const Ctx& c, double* Q)
     }
 }
 
+/* the scheme's positivity step: Zhang-Shu (DGSEM) or the nodal repair (SD) */
+static void limiter(const Ctx& c, double* Q)
+{
+    if (c.cfg.scheme == DG_SCHEME_SD) sd_repair(c, Q);
+    else zs_limiter(c, Q);
+}
+
 static double max_lambda(const Ctx& c, const double* lam)
 {
     double v = 0.0;   /* reduce_max starts from 0, :647-648 */
@@ -595,6 +811,7 @@ OR_API int oracle_create(const dg_config* cfg, void** out)
     lgl(c->p, c->xi, c->w);
     for (int i = 0; i < c->NN; i++) c->winv[i] = 1.0 / c->w[i];
     build_D(c->NN, c->xi, c->D);
+    build_sd(*c);
     size_t n = (size_t)4 * c->NEL * c->NN2;
     c->Q0.resize(n); c->Q1.resize(n); c->Q2.resize(n); c->RHS.resize(n); c->lam.resize(c->NEL);
     *out = c;
@@ -613,6 +830,18 @@ OR_API int oracle_set_threads(int n) {
 }
 
 OR_API void oracle_lgl(int p, double* x, double* w) { lgl(p, x, w); }
+
+/* SD operators of a context: xf[NF], Isf[NF][NN], Dfs[NN][NF] (row-major) */
+OR_API void oracle_sd_operators(void* ctx, double* xf, double* Isf, double* Dfs)
+{
+    const Ctx& c = *(Ctx*)ctx;
+    for (int k = 0; k < c.NF; k++) {
+        if (xf) xf[k] = c.xf[k];
+        for (int j = 0; j < c.NN; j++) if (Isf) Isf[k * c.NN + j] = c.Isf[k][j];
+    }
+    for (int i = 0; i < c.NN; i++)
+        for (int k = 0; k < c.NF; k++) if (Dfs) Dfs[i * c.NF + k] = c.Dfs[i][k];
+}
 
 OR_API void oracle_build_D(int p, double* Dout)
 {
@@ -690,7 +919,8 @@ OR_API int oracle_init_state(void* ctx, const dg_ic_params* ic, double* Q)
             if (ic->kind == DG_IC_CIRCULAR_SOD) {       /* :692-717 */
                 double cx = c.cfg.x0 + 0.5 * c.cfg.L, cy = c.cfg.y0 + 0.5 * c.cfg.L;
                 double r = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy));
-                double delta = 0.5 * c.h;
+                /* front width: 0.5h (dgsem_lobatto_2d.cu:710), 2h (fr_lobatto_2d.cu:648) */
+                double delta = (c.cfg.scheme == DG_SCHEME_SD ? 2.0 : 0.5) * c.h;
                 double phi = 0.5 * (1.0 + tanh((0.25 * c.cfg.L - r) / delta));
                 W[0] = 0.125 + (ic->rho_hi - 0.125) * phi; W[1] = 0.0; W[2] = 0.0;
                 W[3] = 0.1 + (ic->p_hi - 0.1) * phi;
@@ -709,7 +939,8 @@ OR_API int oracle_init_state(void* ctx, const dg_ic_params* ic, double* Q)
             for (int q = 0; q < 4; q++) Q[sidx(c, q, el, jn, in)] = U[q];
         }
     }
-    if (c.cfg.limiter) limiter(c, Q);
+    /* the SD driver applies no limiter to its IC (fr_lobatto_2d.cu:921-923) */
+    if (c.cfg.limiter && c.cfg.scheme != DG_SCHEME_SD) limiter(c, Q);
     return DG_OK;
 }
 

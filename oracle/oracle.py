@@ -52,6 +52,7 @@ def lib():
         L.oracle_sensor.argtypes = [C.c_void_p, _dp, _dp]
         L.oracle_totals.argtypes = [C.c_void_p, _dp, _dp]
         L.oracle_frame_fields.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.oracle_sd_operators.argtypes = [C.c_void_p, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -182,6 +183,13 @@ class Oracle:
         rho = np.zeros(self.NEL); p = np.zeros(self.NEL)
         lib().oracle_frame_fields(self._ctx, _p(Q), _p(rho), _p(p))
         return rho, p
+
+    def sd_operators(self):
+        """Spectral-difference tables: xf[p+2], Isf[p+2, p+1], Dfs[p+1, p+2]."""
+        nf, nn = self.NN + 1, self.NN
+        xf = np.zeros(nf); Isf = np.zeros(nf * nn); Dfs = np.zeros(nn * nf)
+        lib().oracle_sd_operators(self._ctx, _p(xf), _p(Isf), _p(Dfs))
+        return xf, Isf.reshape(nf, nn), Dfs.reshape(nn, nf)
 
     def totals(self, Q):
         out = np.zeros(5)

@@ -66,3 +66,18 @@ def test_struct_layout_matches_header(tmp_path):
     offs = [int(v) for v in vals[2:]]
     exp = [getattr(capi.DGConfig, f).offset for f in fields_cfg] + [getattr(capi.DGICParams, f).offset for f in fields_ic]
     assert offs == exp
+
+
+def test_scheme_validation_before_any_device_call():
+    """dg_create / dg_batch_create reject invalid spectral-difference
+    configurations during validation, before any HIP call."""
+    lib = load_library()
+    ctx = C.c_void_p()
+    for bad in (capi.sd_config(K=4, p=3, av=True), capi.make_config(K=4, p=3, scheme=2),
+                capi.sd_config(K=4, p=8)):
+        assert lib.dg_create(C.byref(bad), None, C.byref(ctx)) == capi.DG_ERR_INVALID_ARG
+        assert not ctx.value
+    arr = (capi.DGConfig * 1)(capi.sd_config(K=4, p=3, limiter=False))
+    b = C.c_void_p()
+    assert lib.dg_batch_create(arr, 1, None, C.byref(b)) == capi.DG_ERR_INVALID_ARG
+    assert lib.dg_sd_operators(None, None, None, None) == capi.DG_ERR_INVALID_ARG
