@@ -49,8 +49,9 @@ def parse():
                     help="E,X: elements per workgroup, XCD remap (implies --no-autotune)")
     ap.add_argument("--cpu-steps", type=int, default=15,
                     help="SSP-RK3 steps of the CPU oracle baseline (about 10 s on 16 host threads)")
-    ap.add_argument("--workload", default="vortex", choices=["vortex", "kh", "batched", "copy"],
+    ap.add_argument("--workload", default="vortex", choices=["vortex", "kh", "sd", "batched", "copy"],
                     help="vortex: the headline line; kh: Kelvin-Helmholtz (--K, --p, --t-end); "
+                         "sd: spectral-difference sibling on circular Sod (--K, --p, --steps); "
                          "batched: seeded ensemble of vortex instances; copy: device copy probe")
     ap.add_argument("--t-end", type=float, default=1.5)
     ap.add_argument("--no-shock-capturing", action="store_true",
@@ -82,7 +83,7 @@ def time_form(D, torch, cfg, ic, steps, warmup, cfl, autotune):
     return s, Q, ms, t, tuned
 
 
-def cpu_baseline(cfg, ic, steps, cfl):
+def cpu_baseline(cfg, ic, steps, cfl, what="vortex"):
     from oracle import oracle as O
     threads = min(16, os.cpu_count() or 1)
     O.set_threads(threads)
@@ -93,7 +94,7 @@ def cpu_baseline(cfg, ic, steps, cfl):
     wall = time.perf_counter() - t0
     dof = o.size
     return {"value": dof * 3 * steps / wall, "unit": "DOF*stage/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, {steps} SSP-RK3 step(s) of the same {cfg.K}x{cfg.K} p={cfg.p} vortex, "
+            "sample": f"FP64 oracle, {steps} SSP-RK3 step(s) of the same {cfg.K}x{cfg.K} p={cfg.p} {what}, "
                       f"{threads} OpenMP threads, {wall:.1f} s"}
 
 
@@ -155,6 +156,61 @@ def bench_kh(args):
                       "unit": "DOF*stage/s", "dof": dof, **out}))
 
 
+def bench_sd(args):
+    """The spectral-difference sibling (fr_lobatto_2d.cu) on the problem of
+    its own driver: circular Sod on [0,1]^2 with reflecting walls, HLLC
+    faces, the nodal positivity repair after every stage and CFL 0.1/p, for a
+    fixed number of fused SSP-RK3 steps in both forms (state resident in HBM,
+    HIP events around the timed steps).  Mass and energy drift over the run
+    (the walls carry neither) and the mixed-vs-FP64 difference are reported
+    beside the rate; the FP64 oracle runs a bounded sample on the host."""
+    import torch
+    from fvstuff_amd import dgsem as D
+    from fvstuff_amd.capi import make_ic, sd_config
+    K, p = args.K, args.p
+    N = p + 1
+    dof = 4 * K * K * N * N
+    cfl = 0.1 / p
+    ic = make_ic("sod")
+    out, Qs = {}, {}
+    for form in args.forms.split(","):
+        s = D.Solver(sd_config(K=K, p=p, precision=form))
+        tuned = None if args.no_autotune else s.autotune()
+        Q = s.init_state(ic)
+        if args.warmup:
+            s.advance(Q, args.warmup, cfl)
+        Q = s.init_state(ic, Q)
+        T0 = s.totals(Q)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        t, _ = s.advance(Q, args.steps, cfl)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        T1 = s.totals(Q)
+        rate = dof * 3 * args.steps / (ms * 1e-3)
+        out[form] = {"value": rate, "ms_per_step": ms / args.steps, "t": t, "schedule": tuned,
+                     "hbm_GBs_algorithmic": rate * (64.0 / 3.0) / 1e9,
+                     "hbm_frac": rate * (64.0 / 3.0) / 1e9 / HBM_PEAK_GBS,
+                     "mass_drift_rel": (T1[0] - T0[0]) / abs(T0[0]),
+                     "energy_drift_rel": (T1[3] - T0[3]) / abs(T0[3])}
+        Qs[form] = (s, Q)
+        log(f"[bench-sd] {form}: {out[form]}")
+    if "fp64" in Qs and "mixed" in Qs:
+        s64, Q64 = Qs["fp64"]
+        nrm = s64.error_norms(Qs["mixed"][1], Q64)
+        out["mixed_vs_fp64_rel_l2"] = nrm["rel_l2"]
+        out["mixed_vs_fp64_rel_linf"] = nrm["rel_linf"]
+        out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sd_config(K=K, p=p), ic, args.cpu_steps, cfl,
+                                           what="circular Sod (spectral difference)")
+    print(json.dumps({"workload": f"spectral-difference circular Sod {K}x{K} p={p}, walls, HLLC, nodal repair, "
+                                  f"CFL {cfl:.4g}, {args.steps} steps",
+                      "unit": "DOF*stage/s", "dof": dof, **out}))
+
+
 def bench_batched(args):
     """Seeded ensemble of isentropic-vortex instances (K ~ U{16..128},
     p ~ U{3,5,7}, strength ~ U[1,5], Mach ~ U[0.1,0.8]) stepped in one
@@ -207,6 +263,8 @@ def main():
         sys.exit(2)
     if args.workload == "kh":
         return bench_kh(args)
+    if args.workload == "sd":
+        return bench_sd(args)
     if args.workload == "batched":
         return bench_batched(args)
     if args.workload == "copy":

@@ -129,7 +129,7 @@ static void diff_matrix(int n, const double* x, double* D) {
     }
 }
 
-// ── spectral-difference operators (fr_lobatto_2d.cu:76-166, :840-866) ──────
+// ── spectral-difference operators (fr_lobatto_2d.cu:76-166, :840-867) ──────
 // Flux nodes {-1, Gauss-Legendre nodes, +1}: the reference's literal GL
 // nodes for p <= 3 (:84-89), else the roots of L_p by Newton in long double
 // from Chebyshev guesses.  Isf[k][j] = l_j(xf_k) (solution-node Lagrange
@@ -766,7 +766,7 @@ int dg_init_state(dg_ctx* c, const dg_ic_params* ic, double* Q) {
     const size_t S = (size_t)c->NEL * c->N2;
     hipLaunchKernelGGL(ic_kernel, dim3(grid_for(S, BLOCK)), dim3(BLOCK), 0, c->stream, Q, a);
     HIPCK(c, hipGetLastError());
-    // the SD driver applies no limiter to its IC (fr_lobatto_2d.cu:921-923)
+    // the SD driver applies no limiter to its IC (fr_lobatto_2d.cu:922-937)
     if (c->cfg.limiter && c->cfg.scheme != DG_SCHEME_SD) return dg_limit(c, Q);
     return DG_OK;
 }

@@ -82,7 +82,7 @@ typedef struct dg_config {
     int    surface_flux;  /* enum dg_surface_flux                                */
     int    bc;            /* enum dg_bc                                          */
     int    av;            /* Ducros-sensor artificial viscosity on/off (AV_ON)   */
-    int    limiter;       /* Zhang-Shu positivity limiter after each stage       */
+    int    limiter;       /* Zhang-Shu limiter after each stage (SD: nodal repair) */
     int    precision;     /* enum dg_precision                                   */
     double av_cav;        /* AV_CAV (0.25)                                       */
     double av_ksensor;    /* AV_KSENSOR (0.1)                                    */
@@ -130,7 +130,7 @@ int         dg_operators(const dg_ctx* ctx, double* x, double* w, double* D);
  * the solution nodes at the flux nodes) and the derivative matrix
  * Dfs[(p+1)*(p+2)] (row-major, Dfs[i][k] = derivative of the flux nodes'
  * Lagrange basis l^f_k at solution node x_i).  Replaces build_Isf /
- * build_Dfs + cudaMemcpyToSymbol, fr_lobatto_2d.cu:99-166,840-866.  Any
+ * build_Dfs + cudaMemcpyToSymbol, fr_lobatto_2d.cu:99-166,840-867.  Any
  * pointer may be NULL. */
 int         dg_sd_operators(const dg_ctx* ctx, double* xf, double* Isf, double* Dfs);
 
@@ -138,7 +138,7 @@ int         dg_sd_operators(const dg_ctx* ctx, double* xf, double* Isf, double* 
  * positivity limiter if it is enabled (ic_kernel + positivity_limiter,
  * dgsem_lobatto_2d.cu:692-717,1006-1007).  DG_SCHEME_SD: the circular Sod
  * front has the width 2h of fr_lobatto_2d.cu:633-657 (0.5h for DGSEM) and no
- * limiter pass follows (fr_lobatto_2d.cu:921-923). */
+ * limiter pass follows (fr_lobatto_2d.cu:922-937). */
 int         dg_init_state(dg_ctx* ctx, const dg_ic_params* ic, double* Q);
 
 /* du/dt of device state Q into RHS (FP64, same layout) and the per-element
@@ -181,7 +181,8 @@ int         dg_cfl_dt(dg_ctx* ctx, const double* lam_el, double cfl,
  * advancing to an end time.
  * t_inout: simulation time in/out (may be NULL); dt_last: last dt (may be NULL).
  * Returns the first numerical failure (NONFINITE / BLOWUP / DT_UNDERFLOW), in
- * which case the state is left as it was after the last good step. */
+ * which case the state is left as it was after the last good step.
+ * DG_SCHEME_SD: the same loop (fr_lobatto_2d.cu:937-990). */
 int         dg_advance(dg_ctx* ctx, double* Q, int nsteps, double cfl, double t_end,
                        double* t_inout, double* dt_last);
 

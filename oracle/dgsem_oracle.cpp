@@ -158,7 +158,7 @@ static void build_D(int n, const double* x, double D[PMAX + 1][PMAX + 1])
     }
 }
 
-/* ── SD operators (fr_lobatto_2d.cu:76-166, :840-866) ────────────────────
+/* ── SD operators (fr_lobatto_2d.cu:76-166, :840-867) ────────────────────
  * Flux nodes xf = {-1, Gauss-Legendre nodes, +1}, p + 2 of them: the
  * reference's literal GL nodes for p <= 3 (:84-89); for p > 3 the roots of
  * L_p, Newton from Chebyshev guesses in long double.
@@ -939,7 +939,7 @@ OR_API int oracle_init_state(void* ctx, const dg_ic_params* ic, double* Q)
             for (int q = 0; q < 4; q++) Q[sidx(c, q, el, jn, in)] = U[q];
         }
     }
-    /* the SD driver applies no limiter to its IC (fr_lobatto_2d.cu:921-923) */
+    /* the SD driver applies no limiter to its IC (fr_lobatto_2d.cu:922-937) */
     if (c.cfg.limiter && c.cfg.scheme != DG_SCHEME_SD) limiter(c, Q);
     return DG_OK;
 }
