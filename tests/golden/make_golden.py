@@ -14,7 +14,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 
-from fvstuff_amd.capi import make_config, make_ic, vortex_config, kh_config  # noqa: E402
+from fvstuff_amd.capi import make_config, make_ic, vortex_config, kh_config, sd_config  # noqa: E402
 from oracle.oracle import Oracle, lgl, diff_matrix  # noqa: E402
 
 # name -> (config, ic, steps, cfl)
@@ -27,6 +27,10 @@ CASES = {
     "vortex_k4_p7": (vortex_config(K=4, p=7), make_ic("vortex"), 10, 0.25),
     "sod_collocation_k6_p2": (make_config(K=6, p=2, volume_flux="collocation", surface_flux="rusanov"),
                               make_ic("sod"), 10, 0.4),
+    # the spectral-difference sibling (fr_lobatto_2d.cu): its driver's Sod and a periodic vortex
+    "sd_sod_k8_p3": (sd_config(K=8, p=3), make_ic("sod"), 20, 0.1 / 3),
+    "sd_vortex_k5_p5": (sd_config(K=5, p=5, L=10.0, bc="periodic", surface_flux="rusanov", limiter=False),
+                        make_ic("vortex"), 10, 0.2),
 }
 
 
@@ -47,9 +51,12 @@ def main():
     for p in range(1, 8):
         x, w = lgl(p)
         out[f"lgl/x{p}"] = x; out[f"lgl/w{p}"] = w; out[f"lgl/D{p}"] = diff_matrix(p)
+        xf, Isf, Dfs = Oracle(sd_config(K=1, p=p)).sd_operators()
+        out[f"sd/xf{p}"] = xf; out[f"sd/Isf{p}"] = Isf; out[f"sd/Dfs{p}"] = Dfs
     for name in CASES:
         out.update(case_arrays(name))
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dgsem_golden.npz")
+    path = sys.argv[1] if len(sys.argv) > 1 else os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                               "dgsem_golden.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")
 

@@ -22,6 +22,15 @@ def test_oracle_operators_match_golden(p):
     np.testing.assert_array_equal(diff_matrix(p), G[f"lgl/D{p}"])
 
 
+@pytest.mark.parametrize("p", range(1, 8))
+def test_oracle_sd_operators_match_golden(p):
+    from fvstuff_amd.capi import sd_config
+    xf, Isf, Dfs = Oracle(sd_config(K=1, p=p)).sd_operators()
+    np.testing.assert_array_equal(xf, G[f"sd/xf{p}"])
+    np.testing.assert_array_equal(Isf, G[f"sd/Isf{p}"])
+    np.testing.assert_array_equal(Dfs, G[f"sd/Dfs{p}"])
+
+
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_oracle_matches_golden(name):
     cfg, ic, steps, cfl = CASES[name]
