@@ -116,7 +116,9 @@ class FrameWriter:
     """The reference's WRITE_FRAME (:984-1003) on top of a Solver: frame
     fields and sensor on the device, copied once per frame, written as
     <prefix>_NNNN.ppm (rho | p) and <sensor_prefix>_NNNN.ppm, plus the
-    plot_rt binary <prefix>_NNNN.bin when `binary` is set."""
+    plot_rt binary <prefix>_NNNN.bin when `binary` is set.  The
+    spectral-difference driver's WRITE_FRAME (fr_lobatto_2d.cu:904-919)
+    writes the rho | p pair alone: prefix="fr_sod", sensor_prefix=None."""
 
     def __init__(self, solver, outdir=".", prefix="dgsem_sod", sensor_prefix="dgsem_sensor", binary=False):
         self.s = solver
@@ -131,15 +133,17 @@ class FrameWriter:
     def write(self, Q, index):
         import torch
         rho_d, p_d = self.s.frame_fields(Q)
-        sen_d = self.s.sensor(Q)
+        sen_d = self.s.sensor(Q) if self.sensor_prefix else None
         torch.cuda.synchronize()
-        rho = rho_d.cpu().numpy(); p = p_d.cpu().numpy(); sen = sen_d.cpu().numpy()
+        rho = rho_d.cpu().numpy(); p = p_d.cpu().numpy()
+        sen = sen_d.cpu().numpy() if sen_d is not None else None
         if self.rho_range is None:
             self.rho_range = image_range(rho)
             self.p_range = image_range(p)
         base = os.path.join(self.outdir, f"{self.prefix}_{index:04d}")
         write_ppm_pair(base + ".ppm", rho, self.rho_range, p, self.p_range, self.K)
-        write_ppm_sensor(os.path.join(self.outdir, f"{self.sensor_prefix}_{index:04d}.ppm"), sen, self.K)
+        if sen is not None:
+            write_ppm_sensor(os.path.join(self.outdir, f"{self.sensor_prefix}_{index:04d}.ppm"), sen, self.K)
         if self.binary:
             write_rt_bin(base + ".bin", rho, p, self.K)
         return rho, p, sen

@@ -198,3 +198,21 @@ def test_wall_box_conservation(dgsem):
     tot = s.totals(Q)
     for q in (0, 3):                      # mass and energy: walls carry no flux of either
         assert abs(tot[q] - tot0[q]) < 1e-13 * abs(tot0[q]), q
+
+
+def test_frames_like_the_sd_driver(dgsem, tmp_path):
+    """WRITE_FRAME of fr_lobatto_2d.cu: the rho | p pair alone, fr_sod_NNNN.ppm,
+    from per-element means that match the oracle's."""
+    from fvstuff_amd import frames
+    cfg = sd_config(K=16, p=3)
+    o = Oracle(cfg); s = dgsem.Solver(cfg)
+    Q = s.init_state(make_ic("sod"))
+    s.advance(Q, 10, 0.1 / 3)
+    fw = frames.FrameWriter(s, outdir=str(tmp_path), prefix="fr_sod", sensor_prefix=None)
+    rho, p, sen = fw.write(Q, 1)
+    assert sen is None and sorted(f.name for f in tmp_path.iterdir()) == ["fr_sod_0001.ppm"]
+    w, h, _ = frames.read_ppm(str(tmp_path / "fr_sod_0001.ppm"))
+    assert (w, h) == (32, 16)
+    rho_o, p_o = o.frame_fields(cpu(Q))
+    np.testing.assert_allclose(rho, rho_o, rtol=1e-14)
+    np.testing.assert_allclose(p, p_o, rtol=1e-13, atol=1e-15)
