@@ -69,12 +69,15 @@ def test_struct_layout_matches_header(tmp_path):
 
 
 def test_scheme_validation_before_any_device_call():
-    """dg_create / dg_batch_create reject invalid spectral-difference
-    configurations during validation, before any HIP call."""
+    """dg_create / dg_batch_create reject invalid configurations (empty grid,
+    degree 0, bad domain or gas, AV or an unknown scheme with spectral
+    difference, batched spectral difference) during validation, before any
+    HIP call."""
     lib = load_library()
     ctx = C.c_void_p()
     for bad in (capi.sd_config(K=4, p=3, av=True), capi.make_config(K=4, p=3, scheme=2),
-                capi.sd_config(K=4, p=8)):
+                capi.sd_config(K=4, p=8), capi.make_config(K=0, p=3), capi.make_config(K=4, p=0),
+                capi.make_config(K=4, p=3, L=0.0), capi.make_config(K=4, p=3, gamma=1.0)):
         assert lib.dg_create(C.byref(bad), None, C.byref(ctx)) == capi.DG_ERR_INVALID_ARG
         assert not ctx.value
     arr = (capi.DGConfig * 1)(capi.sd_config(K=4, p=3, limiter=False))
