@@ -133,6 +133,17 @@ __device__ __forceinline__ void st_at(gptr base, unsigned byte_off, double v) {
 __device__ __forceinline__ double ld_stream(gcptr base, unsigned byte_off) {
     return __builtin_nontemporal_load((gcptr)((DG_GLOBAL const char*)base + byte_off));
 }
+// the same for two adjacent FP64 values (16-byte aligned offsets)
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ dbl2 ld2_at(gcptr base, unsigned byte_off) {
+    return *(DG_GLOBAL const dbl2*)((DG_GLOBAL const char*)base + byte_off);
+}
+__device__ __forceinline__ dbl2 ld2_stream(gcptr base, unsigned byte_off) {
+    return __builtin_nontemporal_load((DG_GLOBAL const dbl2*)((DG_GLOBAL const char*)base + byte_off));
+}
+__device__ __forceinline__ void st2_at(gptr base, unsigned byte_off, dbl2 v) {
+    __builtin_nontemporal_store(v, (DG_GLOBAL dbl2*)((DG_GLOBAL char*)base + byte_off));
+}
 
 // (row, column) of element el of a K x K grid without an integer division:
 // the FP64 quotient is within one of the true one for el < 2^31.
@@ -488,7 +499,7 @@ template <int P, int VOL, bool FULL> struct Occ<P, double, VOL, FULL> {
 // single-solve kernels run slower, measured).
 // U0_EARLY: U^n is loaded with the stage input at the start (see below).
 template <int P, typename Real, bool FULL, int VOL, bool PIN = (sizeof(Real) == 8), bool UP_LDS = true,
-          bool U0_EARLY = !FULL>
+          bool U0_EARLY = !FULL, bool PAIR_OK = true>
 __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int tile, bool lead) {
     static_assert(UP_LDS || !FULL, "the limiter needs the stage values in LDS");
     constexpr bool SD = VOL == VOL_SD;
@@ -583,21 +594,43 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
 
     // ── A: load nodes, prims in FP64, store compute-precision prims ──
     // A thread owns at most ceil((p+1)/2) node slots (E(p+1)^2 nodes over at
-    // least 2E(p+1) threads), s = tid + j*blockDim for j < JMAX, here and in
-    // the update phase.  All of its global loads are issued before any
-    // conversion, so each wave keeps many requests in flight.
+    // least 2E(p+1) threads), slot(j) for j < JMAX, here and in the update
+    // phase.  All of its global loads are issued before any conversion, so
+    // each wave keeps many requests in flight.
     constexpr int JMAX = (N + 1) / 2;
+    // PAIR (lean kernels with an even node count per element and an even
+    // slot count, p = 3 and 7): a thread owns its slots in pairs of adjacent
+    // nodes of one element, slot(2m) = 2(tid + m*blockDim) and slot(2m)+1,
+    // and moves each pair with one 16-byte load or store (the state pointers
+    // are 16-byte aligned).  An element's nodes then span N2/2 lanes, which
+    // the DPP element maxima of the shock-capturing and SD kernels do not
+    // assume; those keep slot(j) = tid + j*blockDim.
+    constexpr bool PAIR = PAIR_OK && !FULL && !SD && N2 % 2 == 0 && JMAX % 2 == 0;
+    auto slot = [&](int j) { return PAIR ? 2 * (tid + (j >> 1) * nthr) + (j & 1) : tid + j * nthr; };
     const bool staged = a.mode == MODE_STAGE;
     double Ul[JMAX][4], U0l[JMAX][4], Upl[JMAX][4];
     gcptr upq[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) upq[q] = sbase(a.Up + tile_off + q * S);
+    if (PAIR) {
 #pragma unroll
-    for (int j = 0; j < JMAX; j++) {
-        const int s = tid + j * nthr;
-        const unsigned sl = s < nE * N2 ? s : 0;     // slots past the tile read node 0
+        for (int j = 0; j < JMAX; j += 2) {
+            const int s = slot(j);
+            const unsigned sl = s < nE * N2 ? s : 0;     // pairs past the tile read nodes 0, 1
 #pragma unroll
-        for (int q = 0; q < 4; q++) Ul[j][q] = ld_at(upq[q], sl * 8u);
+            for (int q = 0; q < 4; q++) {
+                const dbl2 v = ld2_at(upq[q], sl * 8u);
+                Ul[j][q] = v.x; Ul[j + 1][q] = v.y;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < JMAX; j++) {
+            const int s = slot(j);
+            const unsigned sl = s < nE * N2 ? s : 0;     // slots past the tile read node 0
+#pragma unroll
+            for (int q = 0; q < 4; q++) Ul[j][q] = ld_at(upq[q], sl * 8u);
+        }
     }
     // U^n for the update (stages 2, 3).  With U0_EARLY (the lean kernels) it
     // is issued with the stage input, so that the update never waits for
@@ -609,12 +642,25 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         gcptr u0q[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) u0q[q] = sbase(src + tile_off + q * S);
+        if (PAIR) {
 #pragma unroll
-        for (int j = 0; j < JMAX; j++) {
-            const int s = tid + j * nthr;
-            const unsigned sl = s < nE * N2 ? s : 0;
+            for (int j = 0; j < JMAX; j += 2) {
+                const int s = slot(j);
+                const unsigned sl = s < nE * N2 ? s : 0;
 #pragma unroll
-            for (int q = 0; q < 4; q++) U0l[j][q] = ld_stream(u0q[q], sl * 8u);
+                for (int q = 0; q < 4; q++) {
+                    const dbl2 v = ld2_stream(u0q[q], sl * 8u);
+                    U0l[j][q] = v.x; U0l[j + 1][q] = v.y;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < JMAX; j++) {
+                const int s = slot(j);
+                const unsigned sl = s < nE * N2 ? s : 0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) U0l[j][q] = ld_stream(u0q[q], sl * 8u);
+            }
         }
     };
     // Issued in every stage (without U^n, in stage 1 and RHS mode, the stage
@@ -690,7 +736,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     }
 #pragma unroll
     for (int j = 0; j < JMAX; j++) {
-        const int s = tid + j * nthr;
+        const int s = slot(j);
         if (s >= E * N2) continue;
         const int e = s / N2, nd = s - e * N2;
         double W[4] = {1.0, 0.0, 0.0, 1.0}, ri = 1.0;
@@ -860,7 +906,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     if (!UP_LDS && staged) {
 #pragma unroll
         for (int j = 0; j < JMAX; j++) {
-            const int s = tid + j * nthr;
+            const int s = slot(j);
             if (s < nE * N2) {
 #pragma unroll
                 for (int q = 0; q < 4; q++) Upl[j][q] = ld_at(upq[q], (unsigned)s * 8u);
@@ -907,17 +953,15 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     gptr outq[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) outq[q] = sbase(a.out + tile_off + q * S);
-#pragma unroll
-    for (int j = 0; j < JMAX; j++) {
-        const int s = tid + j * nthr;
-        if (s >= nE * N2) continue;
+    // stage value of slot j (node s of the tile): Shu-Osher combination,
+    // sanitize_cons, the SD repair
+    auto stage_value = [&](int j, int s, double (&Us)[4]) {
         const int e = s / N2, nd = s - e * N2;
         const Real* r = sR + e * 4 * N2 + nd;
-        double* up = sU + e * 4 * N2 + nd;
+        const double* up = sU + e * 4 * N2 + nd;
         double Ui[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) Ui[q] = UP_LDS ? up[q * N2] : Upl[j][q];
-        double Us[4];
         if (a.ca != 0.0) {
 #pragma unroll
             for (int q = 0; q < 4; q++) Us[q] = a.ca * U0l[j][q] + a.cb * (Ui[q] + dt * (double)r[q * N2]);
@@ -927,16 +971,39 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         }
         Cons64::sanitize_cons(Us, gs);
         if (SD && a.limiter) Cons64::repair(Us, gs, a.eps_rho, a.eps_p);
-        if (lim) {
+    };
+    auto speed = [&](const double (&Us)[4]) {
+        double W[4], ri;
+        Cons64::to_prim(Us, W, gs, ri);
+        return SD ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri);
+    };
+    if (PAIR) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) up[q * N2] = Us[q];
-        } else {
+        for (int j = 0; j < JMAX; j += 2) {
+            const int s = slot(j);
+            if (s >= nE * N2) continue;
+            double A[4], B[4];
+            stage_value(j, s, A);
+            stage_value(j + 1, s + 1, B);
 #pragma unroll
-            for (int q = 0; q < 4; q++) st_at(outq[q], (unsigned)s * 8u, Us[q]);
-            if (want_lambda) {
-                double W[4], ri;
-                Cons64::to_prim(Us, W, gs, ri);
-                lmax = nanmax(lmax, SD ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri));
+            for (int q = 0; q < 4; q++) st2_at(outq[q], (unsigned)s * 8u, dbl2{A[q], B[q]});
+            if (want_lambda) lmax = nanmax(nanmax(lmax, speed(A)), speed(B));
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < JMAX; j++) {
+            const int s = slot(j);
+            if (s >= nE * N2) continue;
+            double Us[4];
+            stage_value(j, s, Us);
+            if (lim) {
+                const int e = s / N2, nd = s - e * N2;
+#pragma unroll
+                for (int q = 0; q < 4; q++) sU[e * 4 * N2 + q * N2 + nd] = Us[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) st_at(outq[q], (unsigned)s * 8u, Us[q]);
+                if (want_lambda) lmax = nanmax(lmax, speed(Us));
             }
         }
     }
@@ -988,6 +1055,9 @@ template <typename Real> struct BatchOcc {
     static constexpr int waves = sizeof(Real) == 4 ? 2 : 1;
     static constexpr bool up_lds = sizeof(Real) == 8;     // stage input kept in LDS (FP64 only)
     static constexpr bool u0_early = sizeof(Real) == 8;   // U^n loaded at the start (FP64 only)
+    // node slots in pairs with 16-byte accesses (see stage_tile): FP64 1.4 %
+    // faster, FP32 2.4 % slower (measured, same box and ensemble)
+    static constexpr bool pair = sizeof(Real) == 8;
 };
 
 template <typename Real, bool FULL, int VOL>
@@ -1000,19 +1070,19 @@ __global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_ke
     const bool lead = tl.y == 0;
     switch (a.P) {
         case 1: stage_tile<1, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
+                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
         case 2: stage_tile<2, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
+                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
         case 3: stage_tile<3, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
+                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
         case 4: stage_tile<4, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
+                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
         case 5: stage_tile<5, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
+                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
         case 6: stage_tile<6, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
+                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
         default: stage_tile<7, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early>(a, sc, tl.y, lead); break;
+                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
     }
 }
 

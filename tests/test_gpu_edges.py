@@ -72,3 +72,19 @@ def test_tiny_grids(dgsem, scheme, bc, K, p):
     t, _ = s.advance(Q, 3, 0.1)
     assert t == pytest.approx(t_ref, rel=1e-13)
     assert rel_l2(cpu(Q), Q_ref, w2) < 1e-12
+
+
+def test_misaligned_state_rejected(dgsem):
+    """The stage kernels move node pairs with 16-byte accesses: a state
+    pointer that is only 8-byte aligned is rejected before any launch."""
+    cfg = vortex_config(K=4, p=3)
+    s = dgsem.Solver(cfg)
+    buf = torch.zeros(s.size + 1, dtype=torch.float64, device="cuda")
+    Q = buf[1:]
+    assert Q.data_ptr() % 16 == 8
+    with pytest.raises(dgsem.DGError) as e:
+        s.advance(Q, 1, 0.1)
+    assert e.value.status == 1
+    with pytest.raises(dgsem.DGError) as e:
+        s.compute_rhs(Q)
+    assert e.value.status == 1
