@@ -217,27 +217,27 @@ __device__ __forceinline__ unsigned long long wave_umax(unsigned long long v) {
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// Element maximum into LDS for node slots that are element-contiguous across
-// lanes (slot s = element * N2 + node): when an element's N2 nodes fill whole
-// DPP rows (N2 = 4, 16) or the whole wave (N2 = 64), the group is reduced
-// with DPP and one lane per element issues the LDS atomic (nodes of one
-// element would otherwise serialise on the same address); other degrees use
-// one atomic per node.  All lanes of the wave must take part; `live` is
-// uniform over an element's group.
-template <int N2>
+// Element maximum into LDS for values of one element held by G consecutive
+// lanes (one node per lane, G = N2; or a node pair per lane, G = N2/2): when
+// the group is 4, 8, 16 or 32 lanes or the whole wave, it is reduced with
+// DPP (row shifts within a row of 16, then a row broadcast for 32) and one
+// lane per element issues the LDS atomic (nodes of one element would
+// otherwise serialise on the same address); other group sizes use one
+// atomic per lane.  All lanes of the wave must take part; `live` is uniform
+// over an element's group.
+template <int G>
 __device__ __forceinline__ void elem_max_lds(unsigned long long* dst, unsigned long long v, bool live) {
     const int lane = threadIdx.x & 63;
-    if (N2 == 64) {
+    if (G == 64) {
         v = wave_umax_to_last(v);
         if (live && lane == 63) atomicMax(dst, v);
-    } else if (N2 == 16 || N2 == 4) {
-        v = umax_dpp_step<0x111, 0xf>(v);
-        v = umax_dpp_step<0x112, 0xf>(v);
-        if (N2 == 16) {
-            v = umax_dpp_step<0x114, 0xf>(v);
-            v = umax_dpp_step<0x118, 0xf>(v);
-        }
-        if (live && (lane & (N2 - 1)) == N2 - 1) atomicMax(dst, v);
+    } else if (G == 4 || G == 8 || G == 16 || G == 32) {
+        v = umax_dpp_step<0x111, 0xf>(v);                    // row_shr:1
+        v = umax_dpp_step<0x112, 0xf>(v);                    // row_shr:2
+        if (G >= 8) v = umax_dpp_step<0x114, 0xf>(v);        // row_shr:4
+        if (G >= 16) v = umax_dpp_step<0x118, 0xf>(v);       // row_shr:8
+        if (G == 32) v = umax_dpp_step<0x142, 0xa>(v);       // row_bcast:15 into rows 1 and 3
+        if (live && (lane & (G - 1)) == G - 1) atomicMax(dst, v);
     } else if (live) {
         atomicMax(dst, v);
     }
@@ -598,14 +598,13 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     // phase.  All of its global loads are issued before any conversion, so
     // each wave keeps many requests in flight.
     constexpr int JMAX = (N + 1) / 2;
-    // PAIR (lean kernels with an even node count per element and an even
-    // slot count, p = 3 and 7): a thread owns its slots in pairs of adjacent
-    // nodes of one element, slot(2m) = 2(tid + m*blockDim) and slot(2m)+1,
-    // and moves each pair with one 16-byte load or store (the state pointers
-    // are 16-byte aligned).  An element's nodes then span N2/2 lanes, which
-    // the DPP element maxima of the shock-capturing and SD kernels do not
-    // assume; those keep slot(j) = tid + j*blockDim.
-    constexpr bool PAIR = PAIR_OK && !FULL && !SD && N2 % 2 == 0 && JMAX % 2 == 0;
+    // PAIR (an even node count per element and an even slot count, p = 3
+    // and 7): a thread owns its slots in pairs of adjacent nodes of one
+    // element, slot(2m) = 2(tid + m*blockDim) and slot(2m)+1, and moves each
+    // pair with one 16-byte load or store (the state pointers are 16-byte
+    // aligned).  An element's nodes then span N2/2 lanes: the element wave
+    // speed folds a lane's two nodes first and reduces over N2/2 lanes.
+    constexpr bool PAIR = PAIR_OK && N2 % 2 == 0 && JMAX % 2 == 0;
     auto slot = [&](int j) { return PAIR ? 2 * (tid + (j >> 1) * nthr) + (j & 1) : tid + j * nthr; };
     const bool staged = a.mode == MODE_STAGE;
     double Ul[JMAX][4], U0l[JMAX][4], Upl[JMAX][4];
@@ -734,6 +733,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         if (tid < E) { sLamE[tid] = 0ull; sThE[tid] = 0ull; }
         __syncthreads();
     }
+    unsigned long long lam_pair = 0ull;   // PAIR: wave speed of the pair's first node
 #pragma unroll
     for (int j = 0; j < JMAX; j++) {
         const int s = slot(j);
@@ -751,9 +751,19 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         Real* p = sW + s * 5;
         p[0] = (Real)0.5 * w.r; p[1] = (Real)0.5 * w.u; p[2] = (Real)0.5 * w.v;
         p[3] = (Real)0.5 * w.p; p[4] = (Real)0.5 * w.z;
-        if ((FULL || SD) && need_lam)
-            elem_max_lds<N2>(&sLamE[e], dbits(SD ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri)),
-                             s < nE * N2);
+        if ((FULL || SD) && need_lam) {
+            const unsigned long long sp =
+                dbits(SD ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri));
+            if (!PAIR) {
+                elem_max_lds<N2>(&sLamE[e], sp, s < nE * N2);
+            } else if (j & 1) {
+                // second node of the pair: fold with the first, reduce the element
+                const unsigned long long m = sp > lam_pair ? sp : lam_pair;
+                elem_max_lds<N2 / 2>(&sLamE[e], m, s < nE * N2);
+            } else {
+                lam_pair = sp;
+            }
+        }
     }
     __syncthreads();
 
@@ -985,9 +995,18 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             double A[4], B[4];
             stage_value(j, s, A);
             stage_value(j + 1, s + 1, B);
+            if (lim) {
+                const int e = s / N2, nd = s - e * N2;
 #pragma unroll
-            for (int q = 0; q < 4; q++) st2_at(outq[q], (unsigned)s * 8u, dbl2{A[q], B[q]});
-            if (want_lambda) lmax = nanmax(nanmax(lmax, speed(A)), speed(B));
+                for (int q = 0; q < 4; q++) {
+                    sU[e * 4 * N2 + q * N2 + nd] = A[q];
+                    sU[e * 4 * N2 + q * N2 + nd + 1] = B[q];
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) st2_at(outq[q], (unsigned)s * 8u, dbl2{A[q], B[q]});
+                if (want_lambda) lmax = nanmax(nanmax(lmax, speed(A)), speed(B));
+            }
         }
     } else {
 #pragma unroll
@@ -1009,18 +1028,28 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     }
     if (lim) {
         limit_block<N>(sU, a.w64, nE, g, a.eps_rho, a.eps_p, sBar, sTrho, sTheta);
-        for (int s = tid; s < nE * N2; s += nthr) {
-            const int e = s / N2, nd = s - e * N2;
-            double Us[4];
+        if (PAIR) {
+            for (int s = 2 * tid; s < nE * N2; s += 2 * nthr) {
+                const int e = s / N2, nd = s - e * N2;
+                double A[4], B[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                Us[q] = sU[e * 4 * N2 + q * N2 + nd];
-                st_at(outq[q], (unsigned)s * 8u, Us[q]);
+                for (int q = 0; q < 4; q++) {
+                    A[q] = sU[e * 4 * N2 + q * N2 + nd];
+                    B[q] = sU[e * 4 * N2 + q * N2 + nd + 1];
+                    st2_at(outq[q], (unsigned)s * 8u, dbl2{A[q], B[q]});
+                }
+                if (want_lambda) lmax = nanmax(nanmax(lmax, speed(A)), speed(B));
             }
-            if (want_lambda) {
-                double W[4], ri;
-                Cons64::to_prim(Us, W, gs, ri);
-                lmax = nanmax(lmax, Cons64::wave_speed(W, g, ri));
+        } else {
+            for (int s = tid; s < nE * N2; s += nthr) {
+                const int e = s / N2, nd = s - e * N2;
+                double Us[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    Us[q] = sU[e * 4 * N2 + q * N2 + nd];
+                    st_at(outq[q], (unsigned)s * 8u, Us[q]);
+                }
+                if (want_lambda) lmax = nanmax(lmax, speed(Us));
             }
         }
     }
