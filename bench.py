@@ -83,19 +83,44 @@ def time_form(D, torch, cfg, ic, steps, warmup, cfl, autotune):
     return s, Q, ms, t, tuned
 
 
-def cpu_baseline(cfg, ic, steps, cfl, what="vortex"):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(cfg, ic, steps, cfl, what="vortex", single_steps=1):
+    """The FP64 oracle on the host: `steps` SSP-RK3 steps on up to 16 OpenMP
+    threads (the reported baseline), and `single_steps` on one thread (in
+    "single_thread"); both bounded samples of the same workload."""
     from oracle import oracle as O
     threads = min(16, os.cpu_count() or 1)
-    O.set_threads(threads)
     o = O.Oracle(cfg)
-    Q = o.init_state(ic)
-    t0 = time.perf_counter()
-    rc, t, dt = o.advance(Q, steps, cfl)
-    wall = time.perf_counter() - t0
     dof = o.size
-    return {"value": dof * 3 * steps / wall, "unit": "DOF*stage/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, {steps} SSP-RK3 step(s) of the same {cfg.K}x{cfg.K} p={cfg.p} {what}, "
-                      f"{threads} OpenMP threads, {wall:.1f} s"}
+    rates = {}
+    for n, k in ((threads, steps), (1, single_steps)):
+        if k <= 0:
+            continue
+        O.set_threads(n)
+        Q = o.init_state(ic)
+        t0 = time.perf_counter()
+        o.advance(Q, k, cfl)
+        wall = time.perf_counter() - t0
+        rates[n] = (dof * 3 * k / wall, k, wall)
+    model = cpu_model()
+    v, k, wall = rates[threads]
+    out = {"value": v, "unit": "DOF*stage/s", "cores": threads, "kind": "port",
+           "sample": f"FP64 oracle, {k} SSP-RK3 step(s) of the same {cfg.K}x{cfg.K} p={cfg.p} {what}, "
+                     f"{threads} OpenMP threads on {model}, {wall:.1f} s"}
+    if 1 in rates:
+        v1, k1, w1 = rates[1]
+        out["single_thread"] = {"value": v1, "cores": 1, "sample": f"{k1} step(s) on 1 thread, {w1:.1f} s"}
+    return out
 
 
 def bench_kh(args):
