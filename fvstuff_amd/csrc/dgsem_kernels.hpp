@@ -982,6 +982,13 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         Cons64::sanitize_cons(Us, gs);
         if (SD && a.limiter) Cons64::repair(Us, gs, a.eps_rho, a.eps_p);
     };
+    // a stage value the Zhang-Shu limiter may change: rho < eps_rho, or a
+    // pressure within a rounding margin of eps_p (NaN counts as such)
+    bool lim_bad = false;
+    auto needs_limit = [&](const double (&Us)[4]) {
+        return !(Us[0] >= a.eps_rho) ||
+               !(Cons64::pressure_from_cons(Us, gs) >= 2.0 * a.eps_p + 1e-13 * fabs(Us[3]));
+    };
     auto speed = [&](const double (&Us)[4]) {
         double W[4], ri;
         Cons64::to_prim(Us, W, gs, ri);
@@ -1002,6 +1009,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                     sU[e * 4 * N2 + q * N2 + nd] = A[q];
                     sU[e * 4 * N2 + q * N2 + nd + 1] = B[q];
                 }
+                lim_bad = lim_bad || needs_limit(A) || needs_limit(B);
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; q++) st2_at(outq[q], (unsigned)s * 8u, dbl2{A[q], B[q]});
@@ -1019,6 +1027,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                 const int e = s / N2, nd = s - e * N2;
 #pragma unroll
                 for (int q = 0; q < 4; q++) sU[e * 4 * N2 + q * N2 + nd] = Us[q];
+                lim_bad = lim_bad || needs_limit(Us);
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; q++) st_at(outq[q], (unsigned)s * 8u, Us[q]);
@@ -1027,7 +1036,14 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         }
     }
     if (lim) {
-        limit_block<N>(sU, a.w64, nE, g, a.eps_rho, a.eps_p, sBar, sTrho, sTheta);
+        // With every node at rho >= eps_rho and p >= 2 eps_p (+ a relative
+        // margin), the limiter's theta is 1 in every element of the tile and
+        // it returns Ubar + (U - Ubar), U up to rounding: such a tile (on
+        // smooth and on most shocked data, every tile) skips it.  The
+        // workgroup reduction is also the barrier before the stage values
+        // are read back from LDS.
+        if (__syncthreads_or(lim_bad))
+            limit_block<N>(sU, a.w64, nE, g, a.eps_rho, a.eps_p, sBar, sTrho, sTheta);
         if (PAIR) {
             for (int s = 2 * tid; s < nE * N2; s += 2 * nthr) {
                 const int e = s / N2, nd = s - e * N2;
