@@ -74,15 +74,15 @@ struct KArgs {
     int check_blowup;    // stage 1: the wave speed in slot_cur came from a step of this call
     double h, gamma, ca, cb, cfl, t_end, av_cav, av_k, eps_rho, eps_p;
     // derived constants, computed once on the host (set_derived): 2/h,
-    // g-1, 1/(g-1), 1/K, the energy floor 1e-14/(g-1); FP32 copies for the
-    // mixed form
-    double jac, gm1, inv_gm1, inv_K, efloor;
-    float jac_f, g_f, gm1_f, inv_gm1_f;
+    // g-1, 1/(g-1), 1/K, the energy floor 1e-14/(g-1), the AV length
+    // h/(2p+1) and its inverse square; FP32 copies for the mixed form
+    double jac, gm1, inv_gm1, inv_K, efloor, len_p, dx2inv;
+    float jac_f, g_f, gm1_f, inv_gm1_f, len_p_f, dx2inv_f;
     int off_U, off_av, off_lam, off_el;   // byte offsets of the LDS regions
 };
 
-// Host side: fill the derived constants of an argument block whose h and
-// gamma are set (the same IEEE divisions the device would do).
+// Host side: fill the derived constants of an argument block whose h, gamma,
+// K and P are set (the same IEEE divisions the device would do).
 inline void set_derived(KArgs& a) {
     a.jac = 2.0 / a.h;
     a.gm1 = a.gamma - 1.0;
@@ -93,6 +93,10 @@ inline void set_derived(KArgs& a) {
     a.g_f = (float)a.gamma;
     a.gm1_f = (float)a.gm1;
     a.inv_gm1_f = (float)a.inv_gm1;
+    a.len_p = a.h / (2 * a.P + 1);
+    a.dx2inv = 1.0 / (a.len_p * a.len_p);
+    a.len_p_f = (float)a.len_p;
+    a.dx2inv_f = 1.0f / (a.len_p_f * a.len_p_f);
 }
 
 // a derived constant in the compute precision (no conversion for FP32)
@@ -366,8 +370,8 @@ __device__ __forceinline__ void pin_pair(Real (&acc)[N][4], int i, int k) {
 // Dv is the volume matrix (D scaled by the volume term's factor).  acc[0] and
 // acc[N-1] hold their first terms already; node k of 0 < k < N-1 receives its
 // first term from the pair (0, k), which initialises it.
-template <int N, typename Real, int VOL, bool PIN>
-__device__ __forceinline__ void line_pairs(const Phys<Real>& ph, const Prim<Real> (&hw)[N], const Real* Dv,
+template <int N, typename Real, int VOL, bool PIN, typename PH>
+__device__ __forceinline__ void line_pairs(const PH& ph, const Prim<Real> (&hw)[N], const Real* Dv,
                                            Real (&acc)[N][4]) {
     Real c[4] = {0, 0, 0, 0};
     if (VOL == VOL_RANOCHA) {
@@ -379,7 +383,7 @@ __device__ __forceinline__ void line_pairs(const Phys<Real>& ph, const Prim<Real
 #pragma unroll
         for (int k = i + 1; k < N; k++) {
             Real F[4];
-            if (VOL == VOL_RANOCHA) Phys<Real>::ranocha_half(hw[i], hw[k], c, ph.inv_gm1, F);
+            if (VOL == VOL_RANOCHA) PH::ranocha_half(hw[i], hw[k], c, ph.inv_gm1, F);
             else ph.template chandrashekar<(N <= 6)>(doubled(hw[i]), doubled(hw[k]), F);
             const Real dik = Dv[i * N + k], dki = Dv[k * N + i];
 #pragma unroll
@@ -395,8 +399,8 @@ __device__ __forceinline__ void line_pairs(const Phys<Real>& ph, const Prim<Real
 
 // ── volume contribution of one line plus its two face terms:
 //    acc[i] = sum_k Dv_ik F#(U_i,U_k) + [i = 0] fm + [i = N-1] fp ──────────
-template <int N, typename Real, int VOL, bool PIN>
-__device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Real> (&hw)[N], const Real* Dv,
+template <int N, typename Real, int VOL, bool PIN, typename PH>
+__device__ __forceinline__ void line_volume(const PH& ph, const Prim<Real> (&hw)[N], const Real* Dv,
                                             const Real (&fm)[4], const Real (&fp)[4], Real (&acc)[N][4]) {
     if (VOL == VOL_COLLOCATION) {
         // linear contraction sum_k D_ik f(U_k) (:431-439)
@@ -438,8 +442,8 @@ __device__ __forceinline__ void line_volume(const Phys<Real>& ph, const Prim<Rea
 //    nodes through Dsd = -(2/h) Dfs:
 //    acc[i] = Dsd[i][0] fm + Dsd[i][N] fp + sum_{m=1}^{N-1} Dsd[i][m] f(Isf[m] U)
 //    y-lines read the momenta swapped and run the x-direction code. ──
-template <int N, typename Real>
-__device__ __forceinline__ void sd_line(const Phys<Real>& ph, const double* eU, int lbase, int lstride, bool ydir,
+template <int N, typename Real, typename PH>
+__device__ __forceinline__ void sd_line(const PH& ph, const double* eU, int lbase, int lstride, bool ydir,
                                         const Real* Isf, const Real* Dsd, const Real (&fm)[4],
                                         const Real (&fp)[4], Real (&acc)[N][4]) {
     constexpr int N2 = N * N, NF = N + 1;
@@ -505,7 +509,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     constexpr bool SD = VOL == VOL_SD;
     static_assert(UP_LDS || !SD, "the SD line reads the stage input from LDS");
     constexpr int N = P + 1, N2 = N * N;
-    using Mx = M<Real>;
+    // FP64 quotients through the Newton-refined reciprocal where the kernel
+    // has registers to spare (p <= 5: KH 1024^2 p = 3 FP64 8 % faster, SD
+    // 1.5 %); at p >= 6 its extra live values cost the FP64 kernels a wave
+    // per SIMD (p = 7: 270 registers, 40 % slower), so they divide exactly
+    constexpr bool NEWTON = sizeof(Real) == 8 && P <= 5;
+    using Mx = M<Real, NEWTON>;
     extern __shared__ __align__(16) unsigned char smem[];
     // dt and the go flag go from thread 0 to the block through the first
     // bytes of the prims region, which the node phase writes only after every
@@ -527,7 +536,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     // the tile's first node in every FP64 stream (variable q at + q*S)
     const size_t tile_off = (size_t)el0 * N2;
 
-    const Phys<Real> ph{kconst<Real>(g, a.g_f), kconst<Real>(a.gm1, a.gm1_f), kconst<Real>(a.inv_gm1, a.inv_gm1_f)};
+    const Phys<Real, NEWTON> ph{kconst<Real>(g, a.g_f), kconst<Real>(a.gm1, a.gm1_f),
+                                kconst<Real>(a.inv_gm1, a.inv_gm1_f)};
 
     // LDS regions
     Real* sW = (Real*)smem;                          // [E][N2][5] prims; reused as sR [E][4][N2]
@@ -785,8 +795,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         }
         __syncthreads();
         // Ducros-type sensor per node, max per element (:488-491)
-        const Real len_p = (Real)(a.h / (2 * P + 1));
-        const Real dx2inv = (Real)1 / (len_p * len_p);
+        const Real len_p = kconst<Real>(a.len_p, a.len_p_f);
+        const Real dx2inv = kconst<Real>(a.dx2inv, a.dx2inv_f);
 #pragma unroll
         for (int j = 0; j < JMAX; j++) {
             // every lane takes part (the element reduction is wave-wide);
@@ -796,9 +806,10 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             const int s = live ? s0 : 0;
             const int e = s0 / N2;
             const Real* p = sW + s * 5;
-            Real rho = Mx::fmax_((Real)2 * p[0], (Real)1e-12), c2 = ph.g * ((Real)2 * p[3]) / rho;
+            // (FP32: quotients through the hardware reciprocal, as in the fluxes)
+            Real rho = Mx::fmax_((Real)2 * p[0], (Real)1e-12), c2 = Mx::div(ph.g * ((Real)2 * p[3]), rho);
             Real divu = Mx::fmin_(jac * (sDu[s] + sDv[s]), (Real)0), du2 = divu * divu;
-            Real th = du2 / (du2 + (Real)a.av_k * c2 * dx2inv + (Real)1e-30);
+            Real th = Mx::div(du2, du2 + (Real)a.av_k * c2 * dx2inv + (Real)1e-30);
             elem_max_lds<N2>(&sThE[e < E ? e : 0], dbits((double)th), live);
         }
         __syncthreads();

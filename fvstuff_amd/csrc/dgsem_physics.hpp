@@ -5,8 +5,12 @@
 // EC flux, HLLC, the wall ghost state) plus two closed-form additions that the
 // reference does not contain: the Ranocha EC/KEP flux and the Rusanov flux.
 //
-// Real = double: every operation is IEEE FP64 (divides and logs are the
-// compiler's correctly rounded / OCML sequences).
+// Real = double: FP64 throughout.  Quotients are IEEE divisions, or, with
+// NEWTON, a * (1/b) with the reciprocal from the hardware estimate
+// (v_rcp_f64) refined by two Newton steps (within about 1 ulp for the
+// normal-range operands here: densities and pressures are floored at 1e-14;
+// about half the instructions of the IEEE sequence, but more live registers
+// at a time); logs and square roots are OCML's FP64 routines.
 // Real = float : the mixed form.  Reciprocal, log2 and sqrt use the gfx950
 // transcendental unit (v_rcp_f32, v_log_f32, v_sqrt_f32, about 1 ulp).
 #pragma once
@@ -15,11 +19,22 @@
 
 namespace dg {
 
-template <typename Real> struct M;
+template <typename Real, bool NEWTON = false> struct M;
 
-template <> struct M<double> {
-    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
-    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+// 1/x for x in the normal range: the hardware reciprocal estimate plus two
+// Newton steps (within about 1 ulp of 1/x; no denormal or overflow scaling,
+// which the IEEE division sequence adds).
+__device__ __forceinline__ double rcp_newton(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+template <bool NEWTON> struct M<double, NEWTON> {
+    static __device__ __forceinline__ double rcp(double x) { return NEWTON ? rcp_newton(x) : 1.0 / x; }
+    static __device__ __forceinline__ double div(double a, double b) { return NEWTON ? a * rcp_newton(b) : a / b; }
     static __device__ __forceinline__ double ln(double x) { return log(x); }
     static __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
     static __device__ __forceinline__ double fmax_(double a, double b) { return fmax(a, b); }
@@ -29,7 +44,7 @@ template <> struct M<double> {
     static constexpr double series_thr = 1e-4;
 };
 
-template <> struct M<float> {
+template <bool NEWTON> struct M<float, NEWTON> {
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float ln(float x) {
@@ -59,16 +74,8 @@ struct Gas64 {
 
 // ── FP64 state-side helpers (sanitize + prim conversion), :208-229, :347-355 ──
 struct Cons64 {
-    // 1/x for x in the normal range used here ([1e-14, 1e30]): the hardware
-    // reciprocal estimate plus two Newton steps (within ~1 ulp of 1/x; no
-    // denormal / overflow scaling, which the IEEE division sequence adds).
-    static __device__ __forceinline__ double rcp(double x) {
-        double y = __builtin_amdgcn_rcp(x);
-        double e = fma(-x, y, 1.0);
-        y = fma(y, e, y);
-        e = fma(-x, y, 1.0);
-        return fma(y, e, y);
-    }
+    // 1/x for x in the normal range used here ([1e-14, 1e30])
+    static __device__ __forceinline__ double rcp(double x) { return rcp_newton(x); }
     static __device__ __forceinline__ double pressure_from_cons(const double U[4], const Gas64& gs) {
         double rho = fmax(U[0], 1e-14);
         double ke = 0.5 * (U[1] * U[1] + U[2] * U[2]) * rcp(rho);
@@ -147,8 +154,8 @@ __device__ __forceinline__ Prim<Real> make_prim(const double W[4]) {
 }
 
 // ── compute-precision physics ─────────────────────────────────────────────
-template <typename Real> struct Phys {
-    using Mx = M<Real>;
+template <typename Real, bool NEWTON = false> struct Phys {
+    using Mx = M<Real, NEWTON>;
     Real g, gm1, inv_gm1;
 
     __device__ __forceinline__ Real sound_speed(Real p, Real rho) const {   // :184-188
