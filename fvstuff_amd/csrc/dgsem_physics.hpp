@@ -20,6 +20,7 @@
 namespace dg {
 
 template <typename Real, bool NEWTON = false> struct M;
+__device__ __forceinline__ double rcp_newton(double x);
 
 // 1/x for x in the normal range: the hardware reciprocal estimate plus two
 // Newton steps (within about 1 ulp of 1/x; no denormal or overflow scaling,
@@ -247,7 +248,9 @@ template <typename Real, bool NEWTON = false> struct Phys {
     __device__ __forceinline__ static void ranocha_half(const Prim<Real>& L, const Prim<Real>& R, const Real c[4],
                                                         Real inv_gm1, Real F[4]) {
         const Real sr = L.r + R.r, sz = L.z + R.z;          // (rho_L+rho_R)/2, (z_L+z_R)/2
-        const Real rr = Mx::rcp(sr * sz);
+        // the one quotient of the smooth path: the Newton-refined reciprocal
+        // in FP64 at every degree (the log branch keeps the policy's division)
+        const Real rr = sizeof(Real) == 8 ? (Real)rcp_newton((double)(sr * sz)) : Mx::rcp(sr * sz);
         const Real ir = sz * rr, iz = sr * rr;              // 2/(rho_L+rho_R), 2/(z_L+z_R)
         const Real dr = L.r - R.r, dz = L.z - R.z;
         const Real fr = dr * ir, ur = fr * fr;
