@@ -1270,8 +1270,11 @@ int dg_batch_set_options(dg_batch* b, int use_graph, int lanes) {
 int dg_batch_advance(dg_batch* b, double* const* Qs, int nsteps, const double* cfl, double t_end, double* t_inout,
                      int* status_out) {
     if (!b || !Qs || !cfl || nsteps < 0) return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_advance: bad argument");
-    for (int i = 0; i < b->n; i++)
+    for (int i = 0; i < b->n; i++) {
         if (!Qs[i] || !(cfl[i] > 0)) return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_advance: instance %d", i);
+        if (!aligned16(Qs[i]))
+            return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_advance: instance %d state is not 16-byte aligned", i);
+    }
     batch_bind(b, Qs, cfl, t_end);
     BHIPCK(b, hipMemcpyAsync(b->d_args, b->h_args.data(), b->h_args.size() * sizeof(KArgs), hipMemcpyHostToDevice,
                              b->stream));

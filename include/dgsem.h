@@ -12,9 +12,10 @@
  *   el = ej*K + ei,  NN = p+1,  NN2 = NN*NN,  NEL = K*K.
  * The state is always FP64.  DG_MIXED computes the right-hand side in FP32 and
  * keeps the stage accumulation, reductions and positivity checks in FP64.
- * The state passed to dg_compute_rhs and dg_advance / dg_step must be 16-byte
- * aligned (hipMalloc'd memory is; the stage kernels move node pairs with
- * 16-byte accesses): a misaligned Q is rejected with DG_ERR_INVALID_ARG.
+ * The state passed to dg_compute_rhs, dg_advance / dg_step and
+ * dg_batch_advance must be 16-byte aligned (hipMalloc'd memory is; the stage
+ * kernels move node pairs with 16-byte accesses): a misaligned state is
+ * rejected with DG_ERR_INVALID_ARG.
  *
  * Errors: every function returns DG_OK (0) or a positive status code; nothing
  * calls exit() (the reference's CK macro exits, dgsem_lobatto_2d.cu:129-136).

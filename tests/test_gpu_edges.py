@@ -89,3 +89,9 @@ def test_misaligned_state_rejected(dgsem):
     with pytest.raises(dgsem.DGError) as e:
         s.compute_rhs(Q)
     assert e.value.status == 1
+    b = dgsem.Batch([vortex_config(K=4, p=3), vortex_config(K=5, p=5)])
+    buf2 = torch.zeros(b.sizes[1] + 1, dtype=torch.float64, device="cuda")
+    Qs = [torch.zeros(b.sizes[0], dtype=torch.float64, device="cuda"), buf2[1:]]
+    with pytest.raises(dgsem.DGError) as e:
+        b.advance(Qs, 1, [0.1, 0.1])
+    assert e.value.status == 1
