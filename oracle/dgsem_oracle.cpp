@@ -26,8 +26,10 @@
  *                            conservation with an EC surface, convergence order
  *   rk3 .................... exactness on cubic-in-time ODEs
  *   limiter ................ mean preservation, identity on admissible data
- *   sensor / AV ............ parity unpinned beyond free-stream (no reference
- *                            vectors exist for them; see the oracle tests)
+ *   sensor / AV ............ closed forms on a linear compression u = a x
+ *                            (theta per element; the AV term through the SBP
+ *                            identity W^-1 D^T W D U = W^-1 B U' - U'',
+ *                            tests/test_oracle_av.py), free stream
  *   SD sibling ............. (fr_lobatto_2d.cu, tests/test_oracle_sd.py) flux
  *                            nodes = Gauss-Legendre roots and the reference's
  *                            literals, Isf / Dfs exact on polynomials of degree
