@@ -26,10 +26,27 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per stage launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of the
-# stage kernel (scaled by a device copy's readings), keyed by (K, p, form);
-# the runs and the scaling are listed in the named profile.
-PMC_TRAFFIC = {(512, 7, "mixed"): (1.437e9, "profiles/stage_kernel_p7_vortex512.md")}
+# stage kernel (scaled by a device copy's readings), keyed by (K, p, form):
+# (bytes, profile that lists the runs and the scaling, KERNEL_SOURCES hash of
+# the build that was profiled).  A figure whose hash differs from the current
+# sources' is not reported (the kernel changed since it was measured).
+PMC_TRAFFIC = {(512, 7, "mixed"): (1.437e9, "profiles/stage_kernel_p7_vortex512.md", "unmeasured")}
 FP64_PEAK_TFLOPS = 78.6
+# the sources the stage kernels are compiled from
+KERNEL_SOURCES = ("fvstuff_amd/csrc/dgsem_kernels.hpp", "fvstuff_amd/csrc/dgsem_physics.hpp",
+                  "fvstuff_amd/csrc/dgsem_inst.hip", "fvstuff_amd/csrc/dgsem_launch.hpp")
+# SSP-RK3 steps of the accuracy comparison: the headline configuration's count
+ACCURACY_STEPS = 200
+
+
+def kernel_sources_hash():
+    """First 16 hex digits of the SHA-256 over the stage kernels' sources."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
 
 
 def parse():
@@ -94,33 +111,51 @@ def cpu_model():
     return "unknown CPU"
 
 
-def cpu_baseline(cfg, ic, steps, cfl, what="vortex", single_steps=1):
-    """The FP64 oracle on the host: `steps` SSP-RK3 steps on up to 16 OpenMP
-    threads (the reported baseline), and `single_steps` on one thread (in
-    "single_thread"); both bounded samples of the same workload."""
+def cpu_share():
+    """Host threads this process may use: the OpenMP thread count the
+    environment sets (the GPU box gives each GPU a share of its cores and
+    sets it), else the CPUs in the process's affinity mask."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return (int(env) if env.isdigit() and int(env) > 0 else aff), aff
+
+
+def cpu_baseline(cfg, ic, steps, cfl, what="vortex", single_steps=1, keep_state=False):
+    """The FP64 oracle on the host: `steps` SSP-RK3 steps on all the threads
+    of the process's CPU share (the reported baseline) and `single_steps` on
+    one thread ("single_thread"); both bounded samples of the same workload.
+    With keep_state, also returns the oracle's state after `steps` steps."""
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads, aff = cpu_share()
     o = O.Oracle(cfg)
     dof = o.size
     rates = {}
+    kept = None
     for n, k in ((threads, steps), (1, single_steps)):
         if k <= 0:
             continue
         O.set_threads(n)
         Q = o.init_state(ic)
         t0 = time.perf_counter()
-        o.advance(Q, k, cfl)
+        rc, t, _ = o.advance(Q, k, cfl)
         wall = time.perf_counter() - t0
         rates[n] = (dof * 3 * k / wall, k, wall)
+        if keep_state and n == threads:
+            kept = (Q, k, t, rc)
     model = cpu_model()
     v, k, wall = rates[threads]
     out = {"value": v, "unit": "DOF*stage/s", "cores": threads, "kind": "port",
+           "host_cpus": os.cpu_count(), "affinity_cpus": aff, "cpu_model": model,
            "sample": f"FP64 oracle, {k} SSP-RK3 step(s) of the same {cfg.K}x{cfg.K} p={cfg.p} {what}, "
-                     f"{threads} OpenMP threads on {model}, {wall:.1f} s"}
+                     f"{threads} OpenMP threads (the process's CPU share; the host has {os.cpu_count()} CPUs) "
+                     f"on {model}, {wall:.1f} s"}
     if 1 in rates:
         v1, k1, w1 = rates[1]
         out["single_thread"] = {"value": v1, "cores": 1, "sample": f"{k1} step(s) on 1 thread, {w1:.1f} s"}
-    return out
+    return (out, kept) if keep_state else out
 
 
 def bench_kh(args):
@@ -164,7 +199,9 @@ def bench_kh(args):
         ms = e0.elapsed_time(e1)
         T1 = s.totals(Q)
         rate = dof * 3 * steps / (ms * 1e-3)
-        out[form] = {"value": rate, "steps": steps, "t": t, "ms": ms,
+        # the rate counts the steps taken; the timed region also holds the
+        # no-op steps launched past t_end (each still streams the state once)
+        out[form] = {"value": rate, "steps": steps, "steps_launched": s.last_launched, "t": t, "ms": ms,
                      "hbm_GBs_algorithmic": rate * (64.0 / 3.0) / 1e9,
                      "hbm_frac": rate * (64.0 / 3.0) / 1e9 / HBM_PEAK_GBS,
                      "energy_drift_rel": (T1[3] - T0[3]) / abs(T0[3]),
@@ -326,30 +363,51 @@ def main():
 
     acc = {}
     if not args.no_accuracy and "fp64" in sols and "mixed" in sols:
+        # untimed: ACCURACY_STEPS steps of each form from the IC (the
+        # configuration's own step count, whatever --steps is), then the
+        # mixed form again to exactly the FP64 form's end time
         s64, Q64, cfg64 = sols["fp64"]
         s32, Q32, _ = sols["mixed"]
-        t64 = res["fp64"]["t"]
-        # equal-time comparison: run the mixed form again to exactly t64
+        Q64 = s64.init_state(ic, Q64)
+        t64, _ = s64.advance(Q64, ACCURACY_STEPS, args.cfl)
         Q32 = s32.init_state(ic, Q32)
-        s32.advance(Q32, 10 * args.steps, args.cfl, t_end=t64)
+        s32.advance(Q32, 10 * ACCURACY_STEPS, args.cfl, t_end=t64)
         mf = s64.error_norms(Q32, Q64)
         # exact advected vortex (periodic images summed), on the device
         ex = s64.vortex_exact(ic, t64, images=True)
         e64 = s64.error_norms(Q64, ex)
         e32 = s64.error_norms(Q32, ex)
-        acc = {"mixed_vs_fp64_rel_l2": mf["rel_l2"], "mixed_vs_fp64_rel_linf": mf["rel_linf"],
+        acc = {"steps": ACCURACY_STEPS, "t": t64,
+               "mixed_vs_fp64_rel_l2": mf["rel_l2"], "mixed_vs_fp64_rel_linf": mf["rel_linf"],
                "fp64_vs_exact_rel_l2": e64["rel_l2"], "mixed_vs_exact_rel_l2": e32["rel_l2"],
-               "fp64_vs_exact_l2_per_variable": e64["l2"], "t": t64}
-        T = s64.totals(Q64)
+               "fp64_vs_exact_rel_linf": e64["rel_linf"], "mixed_vs_exact_rel_linf": e32["rel_linf"],
+               "fp64_vs_exact_l2_per_variable": e64["l2"]}
         log(f"[bench] accuracy: {acc}")
 
     cpu = None
     if not args.no_cpu:
         cfgc = vortex_config(K=K, p=p, precision="fp64")
-        cpu = cpu_baseline(cfgc, ic, args.cpu_steps, args.cfl)
+        cpu, kept = cpu_baseline(cfgc, ic, args.cpu_steps, args.cfl, keep_state=True)
         log(f"[bench] cpu baseline: {cpu}")
+        if kept is not None and kept[3] == 0:
+            # the full-size GPU states after the oracle's steps, against the oracle
+            import numpy as np
+            Qo, k, to, _ = kept
+            vs = {}
+            for form, (sf, Qf, _) in sols.items():
+                Qf = sf.init_state(ic, Qf)
+                tf, _ = sf.advance(Qf, k, args.cfl)
+                ref = torch.from_numpy(Qo).to(Qf.device)
+                nrm = sf.error_norms(Qf, ref)
+                vs[form] = {"rel_l2": nrm["rel_l2"], "rel_linf": nrm["rel_linf"], "t": tf,
+                            "t_rel_diff": abs(tf - to) / to}
+                del ref
+            acc["vs_oracle"] = {"steps": k, "t_oracle": to, **vs}
+            log(f"[bench] vs oracle after {k} steps: {acc['vs_oracle']}")
 
+    src_hash = kernel_sources_hash()
     traffic = PMC_TRAFFIC.get((K, p, main_form))
+    traffic_ok = traffic is not None and traffic[2] == src_hash
     out = {
         "metric": "DOF*stage updates/s, mixed vs FP64 DGSEM at equal L2 error; % HBM peak",
         "value": res[main_form]["rate"],
@@ -370,9 +428,11 @@ def main():
         "mixed_over_fp64": (res["mixed"]["rate"] / res["fp64"]["rate"]) if ("mixed" in res and "fp64" in res) else None,
         "accuracy": acc,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic[0] if traffic else None,
-                     "traffic_source": traffic[1] if traffic else None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic[0] if traffic_ok else None,
+                     "traffic_source": (f"{traffic[1]} (kernel sources {traffic[2]})" if traffic_ok else
+                                        f"not measured for kernel sources {src_hash}"),
                      "kernel": f"stage_kernel<{p},{'float' if main_form == 'mixed' else 'double'},lean,Ranocha>",
+                     "kernel_sources": src_hash,
                      "bytes_per_dof_stage": b_alg, "avg_launch_ms": ms_launch},
         "cpu_baseline": cpu,
     }

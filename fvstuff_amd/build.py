@@ -1,7 +1,9 @@
 """Builds fvstuff_amd/libdgsem.so for gfx950 with hipcc (in-tree).
 
 One object per polynomial degree (dgsem_inst.hip with -DDG_P=p) plus the C-ABI
-unit, compiled in parallel, linked into one shared library.
+unit, compiled in parallel, linked into one shared library.  Then the C++
+host driver fvstuff_amd/driver/dgsem_run (plain g++ against include/dgsem.h,
+linked to libdgsem.so and the HIP runtime).
 """
 import os
 import subprocess
@@ -13,6 +15,10 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdgsem.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+ROOT = os.path.dirname(HERE)
+DRIVER_SRC = os.path.join(HERE, "driver", "dgsem_run.cpp")
+DRIVER = os.path.join(HERE, "driver", "dgsem_run")
 ARCH = "gfx950"
 DEGREES = range(1, 8)
 
@@ -54,7 +60,21 @@ def build(jobs=None, verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + [u[1] for u in units])
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+def build_driver():
+    """The C++ host driver: a plain host compiler, the ABI header, the
+    library (found next to the binary's parent folder at run time) and the
+    HIP runtime."""
+    tmp = DRIVER + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
+          "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROCM, "include"),
+          DRIVER_SRC, "-o", tmp, "-L", HERE, "-ldgsem", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+          "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

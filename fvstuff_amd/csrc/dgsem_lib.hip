@@ -862,7 +862,10 @@ static int advance_impl(dg_ctx* c, double* Q, int nsteps, double cfl, double t_e
             int r1 = advance_step(c, Q, 1, cfl, t_end, E, xcd, c->cap);
             int r2 = r1 ? r1 : advance_step(c, Q, 2, cfl, t_end, E, xcd, c->cap);
             hipError_t ec = hipStreamEndCapture(c->cap, &g);
-            if (r2) return r2;
+            if (r2) {
+                if (ec == hipSuccess) hipGraphDestroy(g);   // a partial capture is never instantiated
+                return r2;
+            }
             HIPCK(c, ec);
             hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
             hipGraphDestroy(g);
@@ -886,9 +889,15 @@ static int advance_impl(dg_ctx* c, double* Q, int nsteps, double cfl, double t_e
     if (dt_last) *dt_last = h.dt;
     c->last_steps = h.steps;
     if (h.status != ST_OK) return fail(c, h.status, "%s at t=%.9g", dg_status_string(h.status), h.t);
-    if (nsteps > 0 && !h.skip) {   // blow-up check after the last step (:1052-1057)
+    if (h.steps > 0) {
+        // blow-up check of the last step taken (:1052-1057).  The steps taken
+        // are the first h.steps of the call (after t_end every step is a
+        // no-op); step n writes its stage-3 wave speed into slot (n & 1) ^ 1,
+        // so the last one, n = h.steps - 1, wrote slot h.steps & 1, and the
+        // no-op steps after it leave that slot alone.  The next step's stage
+        // 1 makes the same check, but a no-op step skips it.
         double lam;
-        const unsigned long long b = h.lam_bits[nsteps & 1];
+        const unsigned long long b = h.lam_bits[h.steps & 1];
         memcpy(&lam, &b, sizeof(lam));
         if (!(lam > 0.0) || lam > 1e6) return fail(c, DG_ERR_BLOWUP, "blow-up: max wave speed %g at t=%.9g", lam, h.t);
     }
@@ -1172,9 +1181,10 @@ static int batch_status(dg_batch* b, int nsteps, double* t_inout, int* status_ou
     int first = DG_OK;
     for (int i = 0; i < b->n; i++) {
         int st = h[i].status;
-        if (st == ST_OK && nsteps > 0 && !h[i].skip) {
+        if (st == ST_OK && h[i].steps > 0) {
+            // the last step taken wrote slot steps & 1 (see advance_impl)
             double lam;
-            memcpy(&lam, &h[i].lam_bits[nsteps & 1], sizeof(lam));
+            memcpy(&lam, &h[i].lam_bits[h[i].steps & 1], sizeof(lam));
             if (!(lam > 0.0) || lam > 1e6) st = DG_ERR_BLOWUP;
         }
         if (t_inout) t_inout[i] = h[i].t;
@@ -1296,6 +1306,7 @@ int dg_batch_advance(dg_batch* b, double* const* Qs, int nsteps, const double* c
             hipError_t e1 = batch_step(b, StepCtl{1, 0, 1});
             hipError_t e2 = batch_step(b, StepCtl{0, 1, 1});
             hipError_t ec = hipStreamEndCapture(b->cap, &g);
+            if (ec == hipSuccess && (e1 != hipSuccess || e2 != hipSuccess)) hipGraphDestroy(g);
             BHIPCK(b, e1); BHIPCK(b, e2); BHIPCK(b, ec);
             hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
             hipGraphDestroy(g);

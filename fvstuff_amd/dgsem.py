@@ -199,11 +199,14 @@ class Solver:
     def advance_to(self, Q, t_end, cfl, chunk=256, t=0.0, max_steps=10 ** 7, progress=None):
         """Advance until t_end in chunks of fused steps (one host sync per
         chunk); returns (t, steps taken).  progress(t, steps) is called after
-        every chunk if given."""
+        every chunk if given.  self.last_launched holds the steps launched,
+        which include the no-op steps of the last chunk past t_end."""
         steps = 0
+        self.last_launched = 0
         n = min(chunk, 8)            # a short first chunk measures dt
         while t < t_end - 1e-12 * max(1.0, abs(t_end)) and steps < max_steps:
             t, dt = self.advance(Q, n, cfl, t_end=t_end, t=t)
+            self.last_launched += n
             steps += self.steps_taken()
             if progress is not None:
                 progress(t, steps)

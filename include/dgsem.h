@@ -184,8 +184,13 @@ int         dg_cfl_dt(dg_ctx* ctx, const double* lam_el, double cfl,
  * (each still streams the state once), so size nsteps from the last dt when
  * advancing to an end time.
  * t_inout: simulation time in/out (may be NULL); dt_last: last dt (may be NULL).
- * Returns the first numerical failure (NONFINITE / BLOWUP / DT_UNDERFLOW), in
- * which case the state is left as it was after the last good step.
+ * Returns the first numerical failure.  NONFINITE (at the first step of the
+ * call) and DT_UNDERFLOW are found when a step starts, before it changes
+ * anything: Q and *t_inout are those of the last good step.  BLOWUP (max
+ * wave speed not > 0 or > 1e6 after a step, :1052-1057) is found after the
+ * step that blew up has written Q: Q is that step's result and *t_inout
+ * includes its dt (the reference's loop breaks before adding it, :1054-1058).
+ * The last step taken is checked even when the steps after it are no-ops.
  * DG_SCHEME_SD: the same loop (fr_lobatto_2d.cu:937-990). */
 int         dg_advance(dg_ctx* ctx, double* Q, int nsteps, double cfl, double t_end,
                        double* t_inout, double* dt_last);

@@ -84,3 +84,12 @@ def test_scheme_validation_before_any_device_call():
     b = C.c_void_p()
     assert lib.dg_batch_create(arr, 1, None, C.byref(b)) == capi.DG_ERR_INVALID_ARG
     assert lib.dg_sd_operators(None, None, None, None) == capi.DG_ERR_INVALID_ARG
+
+
+def test_driver_built_and_rejects_bad_arguments():
+    """The C++ host driver links the library; bad arguments are rejected
+    before any device call."""
+    from fvstuff_amd.build import DRIVER
+    assert os.path.exists(DRIVER), "driver not built (python -m fvstuff_amd.build)"
+    r = subprocess.run([DRIVER, "--case", "nope"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "usage" in r.stderr

@@ -1,0 +1,180 @@
+"""GPU parity of the mixed shock-capturing stage kernels against the FP64 oracle.
+
+The kernels: stage_kernel<P, float, FULL> — the Ducros sensor and the
+artificial viscosity in FP32 (dgsem_lobatto_2d.cu:483-521), the Chandrashekar
+volume flux and HLLC faces in FP32, the Zhang-Shu limiter in FP64 (:584-643,
+fused into every stage and skipped on tiles whose nodes are all admissible),
+the FP64 stage update and wave-speed reduction.  They run the Kelvin-Helmholtz
+workloads.  Bars (mixed form vs the FP64 oracle): one right-hand side within
+3e-6 relative L2 at p <= 4 and 5e-6 at p = 7 (about 50 FP32 ulps of the RHS
+norm: next to a jump the RHS is a difference of much larger fluxes, and the
+row sums of |D| grow like p^2: 6 at p = 3, 28 at p = 7), per-element wave
+speeds within 1e-6, and 20 fused steps within 1e-6 relative L2, with the
+limiter firing; the FP64 form of the same kernels within 1e-13 (RHS) and
+1e-11 (20 steps through the limiter's bisection).
+
+Degrees 1, 3, 4 and 7 cover every way an element's node maxima are reduced
+(groups of 4, 8 or 16 lanes, a whole wave, one atomic per lane), and a 7x7
+grid with 3-element tiles covers a partial last tile.
+"""
+import numpy as np
+import pytest
+import torch
+
+from fvstuff_amd.capi import kh_config, make_config, make_ic
+from oracle.oracle import Oracle, lgl
+from tests.layout import gll_weights_2d, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dgsem():
+    from fvstuff_amd import dgsem as D
+    D.load_library()
+    assert torch.cuda.is_available()
+    return D
+
+
+def cpu(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy().copy()
+
+
+def gpu(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def w2(cfg):
+    _, w = lgl(cfg.p)
+    return gll_weights_2d(w, cfg.K, cfg.L / cfg.K)
+
+
+def kh_sc(K, p, precision="mixed", **kw):
+    """The Kelvin-Helmholtz workload's configuration: Chandrashekar + HLLC,
+    Ducros AV and the Zhang-Shu limiter, periodic [-1,1]^2."""
+    return kh_config(K=K, p=p, precision=precision, volume_flux="chandrashekar", surface_flux="hllc",
+                     av=True, limiter=True, **kw)
+
+
+def with_precision(cfg, precision):
+    c = type(cfg).from_buffer_copy(cfg)
+    c.precision = 0 if precision == "fp64" else 1
+    return c
+
+
+def sod_moving_state(K, p):
+    """The reference's default run (walls, AV, limiter) after 8 oracle steps
+    (CFL 0.6, the reference's default, at p <= 3; 0.2 at p = 7): the front
+    has started to move, so the sensor and the AV are active."""
+    cfg = make_config(K=K, p=p)
+    o = Oracle(cfg)
+    Q = o.init_state(make_ic("sod"))
+    rc, _, _ = o.advance(Q, 8, 0.6 if p <= 3 else 0.2)
+    assert rc == 0
+    return Q
+
+
+RHS_CASES = [
+    ("kh-p1", kh_sc(12, 1), "kh"),
+    ("kh-p3", kh_sc(8, 3), "kh"),
+    ("kh-p4", kh_sc(8, 4), "kh"),
+    ("kh-p7", kh_sc(5, 7), "kh"),
+    ("kh-p3-ragged-tiles", kh_sc(7, 3, elems_per_block=3), "kh"),
+    ("sod-walls-p3", make_config(K=12, p=3, precision="mixed"), "sod"),
+    ("sod-walls-p7", make_config(K=6, p=7, precision="mixed"), "sod"),
+]
+
+
+def rhs_bar(p):
+    return 3e-6 if p <= 4 else 5e-6
+
+
+@pytest.mark.parametrize("name,cfg,ic", RHS_CASES, ids=[c[0] for c in RHS_CASES])
+def test_mixed_shock_capturing_rhs(dgsem, name, cfg, ic):
+    cfg64 = with_precision(cfg, "fp64")
+    o = Oracle(cfg64)
+    if ic == "kh":
+        Q = o.init_state(make_ic("kh", noise_amp=0.3))
+    else:
+        Q = sod_moving_state(cfg.K, cfg.p)
+    R_ref, lam_ref = o.compute_rhs(Q)
+    # the case exercises the AV: switching it off changes the RHS
+    cfg_noav = with_precision(cfg, "fp64")
+    cfg_noav.av = 0
+    R_noav, _ = Oracle(cfg_noav).compute_rhs(Q)
+    assert rel_l2(R_noav, R_ref, w2(cfg)) > 1e-3
+    s = dgsem.Solver(cfg)
+    R, lam = s.compute_rhs(gpu(Q))
+    err = rel_l2(cpu(R), R_ref, w2(cfg))
+    print(f"{name}: mixed RHS rel L2 vs oracle {err:.2e}")
+    assert err < rhs_bar(cfg.p)
+    np.testing.assert_allclose(cpu(lam), lam_ref, rtol=1e-6)
+    # the FP64 form of the same kernel, as a control
+    R64, lam64 = dgsem.Solver(cfg64).compute_rhs(gpu(Q))
+    assert rel_l2(cpu(R64), R_ref, w2(cfg)) < 1e-13
+
+
+LOOP_CASES = [
+    # a circular blast (pressure ratio 1e6): the limiter changes the state
+    # (with it off the same steps differ by O(1) at p = 3)
+    ("blast-p3", dict(K=10, p=3), 0.4, True, True),
+    ("blast-p1", dict(K=16, p=1), 0.4, False, True),
+    # at p = 7 only the FP64 form is held to the oracle on this blast: the
+    # FP32 flux error at the front (1e6 x 6e-8 = 0.06) is as large as the
+    # low-pressure state next to it (p = 0.1), so the mixed form's limiter
+    # acts on other nodes than the FP64 one's (measured: 3.8e-4 relative L2
+    # after 20 steps); the mixed p = 7 loop is the KH case below
+    ("blast-p7", dict(K=6, p=7), 0.4, False, False),
+]
+
+
+@pytest.mark.parametrize("name,kw,cfl,fires,mixed", LOOP_CASES, ids=[c[0] for c in LOOP_CASES])
+def test_shock_capturing_fused_loop(dgsem, name, kw, cfl, fires, mixed):
+    ic = make_ic("sod", rho_hi=1.0, p_hi=1e6)
+    cfg64 = make_config(**kw)
+    o = Oracle(cfg64)
+    Q_ref = o.init_state(ic)
+    Q0 = Q_ref.copy()
+    rc, t_ref, _ = o.advance(Q_ref, 20, cfl)
+    assert rc == 0
+    if fires:
+        o_off = Oracle(make_config(limiter=False, **kw))
+        Q_off = Q0.copy()
+        assert o_off.advance(Q_off, 20, cfl)[0] == 0
+        assert np.abs(Q_off - Q_ref).max() > 1e-2
+    for precision, tol, ttol in (("fp64", 1e-11, 1e-12), ("mixed", 1e-6, 1e-6)):
+        if precision == "mixed" and not mixed:
+            continue
+        s = dgsem.Solver(make_config(precision=precision, **kw))
+        Q = gpu(Q0)
+        t, _ = s.advance(Q, 20, cfl)
+        err = rel_l2(cpu(Q), Q_ref, w2(cfg64))
+        print(f"{name} {precision}: state rel L2 vs oracle after 20 steps {err:.2e}, t {t:.9g} vs {t_ref:.9g}")
+        assert t == pytest.approx(t_ref, rel=ttol)
+        assert err < tol
+
+
+@pytest.mark.parametrize("p,K", [(3, 16), (7, 6)])
+def test_mixed_kh_fused_loop(dgsem, p, K):
+    """20 fused steps of the KH workload itself (noise 1e-3) in the mixed
+    form against the oracle; mass and energy conserved to FP32 round-off
+    (the FP64 state accumulates FP32 right-hand sides)."""
+    cfg = kh_sc(K, p)
+    cfl = 0.5 if p <= 3 else 0.2
+    o = Oracle(with_precision(cfg, "fp64"))
+    ic = make_ic("kh")
+    Q_ref = o.init_state(ic)
+    rc, t_ref, _ = o.advance(Q_ref, 20, cfl)
+    assert rc == 0
+    s = dgsem.Solver(cfg)
+    Q = s.init_state(ic)
+    T0 = s.totals(Q)
+    t, _ = s.advance(Q, 20, cfl)
+    T1 = s.totals(Q)
+    err = rel_l2(cpu(Q), Q_ref, w2(cfg))
+    print(f"KH p={p} mixed rel L2 vs oracle after 20 steps {err:.2e}")
+    assert t == pytest.approx(t_ref, rel=1e-6)
+    assert err < 1e-6
+    for q in (0, 3):
+        assert abs(T1[q] - T0[q]) <= 1e-7 * abs(T0[q])
