@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/dgsem.h"
@@ -359,16 +360,20 @@ __global__ void repair_kernel(double* Q, size_t S, double g, double eps_rho, dou
     }
 }
 
+// GLL weights of one degree, passed by value (no device table needed)
+struct GllW {
+    double w[16];
+};
+
 // GLL-weighted integrals of rho, rho u, rho v, E and -rho s/(g-1); one
-// partial per block, summed on the host in block order (deterministic).
-__global__ void totals_kernel(const double* Q, const double* w, int N, size_t S, double jw, double g,
-                              double* part) {
+// partial per block into part[block*5 + q] (totals_finalize_kernel sums them).
+__global__ void totals_kernel(const double* Q, GllW w, int N, size_t S, double jw, double g, double* part) {
     __shared__ double sred[5][BLOCK / 64];
     double acc[5] = {0, 0, 0, 0, 0};
     const int N2 = N * N;
     for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < S; n += (size_t)blockDim.x * gridDim.x) {
         const int nd = (int)(n % N2), jn = nd / N, in = nd - jn * N;
-        const double wij = jw * w[in] * w[jn];
+        const double wij = jw * w.w[in] * w.w[jn];
         double U[4] = {Q[n], Q[S + n], Q[2 * S + n], Q[3 * S + n]};
         for (int q = 0; q < 4; q++) acc[q] += wij * U[q];
         const double p = (g - 1.0) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0]);
@@ -390,6 +395,31 @@ __global__ void totals_kernel(const double* Q, const double* w, int N, size_t S,
             for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += sred[q][i];
             part[(size_t)blockIdx.x * 5 + q] = s;
         }
+}
+
+// Sum of the nb per-block partials of totals_kernel into out[0..4], on one
+// workgroup of BLOCK threads in a fixed order (thread t takes blocks t,
+// t + BLOCK, ...; then fixed shuffle and wave orders): deterministic, and no
+// host round trip.  Reads only the nb partials the last totals launch wrote.
+__global__ void totals_finalize_kernel(const double* part, int nb, double* out) {
+    __shared__ double sred[5][BLOCK / 64];
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int q = 0; q < 5; q++) acc[q] += part[(size_t)b * 5 + q];
+    for (int q = 0; q < 5; q++) {
+        double v = acc[q];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        acc[q] = v;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int q = 0; q < 5; q++) sred[q][wid] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double s = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += sred[threadIdx.x][i];
+        out[threadIdx.x] = s;
+    }
 }
 
 // Exact vortex advected for time t by the background velocity: the vortex
@@ -927,20 +957,75 @@ int dg_sensor(dg_ctx* c, const double* Q, double* sensor_el) {
     return DG_OK;
 }
 
+// The totals of state Q on `stream`: totals_kernel's per-block partials in
+// part[0 .. 5*blocks), their sum in out_dev[0..4] (device memory).
+static hipError_t launch_totals(const double* Q, int p, int K, double L, double gamma, double* part,
+                                double* out_dev, hipStream_t stream) {
+    const int N = p + 1;
+    const size_t S = (size_t)K * K * N * N;
+    const double h = L / K;
+    GllW w;
+    double x[16];
+    lgl_nodes(p, x, w.w);
+    const int blocks = grid_for(S, BLOCK);
+    hipLaunchKernelGGL(totals_kernel, dim3(blocks), dim3(BLOCK), 0, stream, Q, w, N, S, 0.25 * h * h, gamma, part);
+    hipLaunchKernelGGL(totals_finalize_kernel, dim3(1), dim3(BLOCK), 0, stream, (const double*)part, blocks, out_dev);
+    return hipGetLastError();
+}
+
 int dg_totals(dg_ctx* c, const double* Q, double out[5]) {
     if (!c || !Q || !out) return fail(c, DG_ERR_INVALID_ARG, "dg_totals: null argument");
-    const size_t S = (size_t)c->NEL * c->N2;
-    hipLaunchKernelGGL(totals_kernel, dim3(c->part_blocks), dim3(BLOCK), 0, c->stream, Q,
-                       (const double*)c->d_tab64 + c->N2, c->N, S, 0.25 * c->h * c->h, c->cfg.gamma, c->d_part);
-    HIPCK(c, hipGetLastError());
-    std::vector<double> part((size_t)c->part_blocks * 5);
-    HIPCK(c, hipMemcpyAsync(part.data(), c->d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // partials in d_part[0, 5*part_blocks), the sums after them
+    double* sums = c->d_part + (size_t)c->part_blocks * 5;
+    HIPCK(c, launch_totals(Q, c->cfg.p, c->cfg.K, c->cfg.L, c->cfg.gamma, c->d_part, sums, c->stream));
+    HIPCK(c, hipMemcpyAsync(out, sums, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
-    for (int q = 0; q < 5; q++) {
-        double s = 0.0;
-        for (int b = 0; b < c->part_blocks; b++) s += part[(size_t)b * 5 + q];
-        out[q] = s;
+    return DG_OK;
+}
+
+// ── totals without allocation (dg_totals_noalloc) ──────────────────────────
+// The per-block partials live in `scratch`, one fixed array per GPU (each GPU
+// holds its own copy until the process ends), sized for the most blocks a
+// totals launch uses (grid_for caps the grid at NOALLOC_BLOCKS).  In host code
+// `scratch` names a host-side stand-in, not the device array: the device
+// address comes from hipGetSymbolAddress, once per device with that device
+// current, and only that address is passed on.  Every call on a GPU shares
+// its array: calls that overlap on one GPU race.  Nothing in it has to start
+// at zero (a call reads only the partials its own totals launch wrote).
+constexpr int NOALLOC_BLOCKS = 8192;
+constexpr int NOALLOC_DEVICES = 64;
+static __device__ __align__(256) double scratch[NOALLOC_BLOCKS * 5];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[NOALLOC_DEVICES] = {};
+
+// device address of the current device's scratch array (looked up once)
+static hipError_t scratch_ptr(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= NOALLOC_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = (double*)ptr;
     }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+int dg_totals_noalloc(const dg_config* cfg, const double* Q, double* out_dev, void* stream) {
+    if (!cfg || !Q || !out_dev) return DG_ERR_INVALID_ARG;
+    if (cfg->K < 1 || cfg->p < 1 || cfg->p > PMAX_GPU || !(cfg->L > 0) || !(cfg->gamma > 1.0) ||
+        (long long)cfg->K * cfg->K > (1LL << 30) / ((cfg->p + 1) * (cfg->p + 1)))
+        return DG_ERR_INVALID_ARG;   // the same limits as dg_create: at most 2^30 nodes
+    const size_t S = (size_t)cfg->K * cfg->K * (cfg->p + 1) * (cfg->p + 1);
+    if (grid_for(S, BLOCK) > NOALLOC_BLOCKS) return DG_ERR_INVALID_ARG;   // more partials than `scratch` holds
+    double* part = nullptr;
+    if (scratch_ptr(&part) != hipSuccess) return DG_ERR_HIP;
+    if (launch_totals(Q, cfg->p, cfg->K, cfg->L, cfg->gamma, part, out_dev, (hipStream_t)stream) != hipSuccess)
+        return DG_ERR_HIP;
     return DG_OK;
 }
 

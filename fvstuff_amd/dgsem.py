@@ -49,6 +49,7 @@ SIGNATURES = {
     "dg_set_graph": (C.c_int, [_vp, C.c_int]),
     "dg_sensor": (C.c_int, [_vp, _vp, _vp]),
     "dg_totals": (C.c_int, [_vp, _vp, _dp]),
+    "dg_totals_noalloc": (C.c_int, [C.POINTER(DGConfig), _vp, _vp, _vp]),
     "dg_vortex_exact": (C.c_int, [_vp, C.POINTER(DGICParams), C.c_double, C.c_int, _vp]),
     "dg_error_norms": (C.c_int, [_vp, _vp, _vp, _dp]),
     "dg_frame_fields": (C.c_int, [_vp, _vp, _vp, _vp]),

@@ -215,6 +215,20 @@ int         dg_sensor(dg_ctx* ctx, const double* Q, double* sensor_el);
  * entropy -rho*s/(gamma-1), s = ln(p*rho^-gamma). */
 int         dg_totals(dg_ctx* ctx, const double* Q, double out[5]);
 
+/* dg_totals without a context and without any device allocation, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The same five integrals as dg_totals, from the same kernels (bit-identical
+ * results), for the state Q of configuration cfg (K, p, L and gamma are
+ * used); written to DEVICE memory out_dev[0..4] on `stream` (a hipStream_t,
+ * NULL = null stream), asynchronously: no host synchronisation, no memset.
+ * The per-block partial sums go to a fixed array of the library (8192 blocks
+ * x 5 doubles = 320 KiB per GPU, held until the process ends); calls that
+ * overlap on one GPU race on it.  Returns DG_ERR_INVALID_ARG for a null
+ * pointer, an invalid configuration or more than 2^30 nodes (dg_create's
+ * limits), DG_ERR_HIP if a launch fails. */
+int         dg_totals_noalloc(const dg_config* cfg, const double* Q, double* out_dev, void* stream);
+
 /* Exact isentropic vortex (vortex_paper_exact, q1_dg_euler_2d.cu:2166-2188,
  * the reference solution of its L2 error kernel :2247-2294) advected for
  * time t >= 0 by the background velocity (u0, v0) on the periodic domain:
