@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--t-end", type=float, default=1.5)
     ap.add_argument("--no-shock-capturing", action="store_true",
                     help="kh: switch off the Ducros AV and the Zhang-Shu limiter")
+    ap.add_argument("--mfma", type=int, default=-1, choices=[-1, 0, 1],
+                    help="kh: the AV contraction on the matrix cores (1), on the VALU (0), or tuned (-1)")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)
     return ap.parse_args()
@@ -177,9 +179,13 @@ def bench_kh(args):
         # layer loses positivity at t ~ 0.15-0.25 (Ranocha or Chandrashekar,
         # Rusanov or HLLC); with it the layer rolls up to t = 1.5
         sc = not args.no_shock_capturing
+        sched = {}
+        if args.schedule:
+            E_, X_ = (int(v) for v in args.schedule.split(","))
+            sched = dict(elems_per_block=E_, xcd_remap=X_)
         s = D.Solver(kh_config(K=K, p=p, precision=form, av=sc, limiter=sc,
-                               volume_flux="chandrashekar", surface_flux="hllc"))
-        if not args.no_autotune:
+                               volume_flux="chandrashekar", surface_flux="hllc", mfma=args.mfma, **sched))
+        if not (args.no_autotune or args.schedule):
             s.autotune()
         Q = s.init_state(ic)
         T0 = s.totals(Q)

@@ -46,7 +46,7 @@ class DGConfig(C.Structure):
         ("av_cav", C.c_double), ("av_ksensor", C.c_double),
         ("eps_rho", C.c_double), ("eps_p", C.c_double),
         ("elems_per_block", C.c_int), ("xcd_remap", C.c_int),
-        ("scheme", C.c_int),
+        ("scheme", C.c_int), ("mfma", C.c_int),
     ]
 
 
@@ -64,7 +64,7 @@ def make_config(K=50, p=3, x0=0.0, y0=0.0, L=1.0, gamma=1.4,
                 volume_flux="chandrashekar", surface_flux="hllc", bc="wall",
                 av=True, limiter=True, precision="fp64", av_cav=0.25,
                 av_ksensor=0.1, eps_rho=1e-6, eps_p=1e-6,
-                elems_per_block=0, xcd_remap=-1, scheme="dgsem"):
+                elems_per_block=0, xcd_remap=-1, scheme="dgsem", mfma=-1):
     """A dg_config; defaults are the reference's compile-time defaults
     (dgsem_lobatto_2d.cu:61-126, Makefile DGSEM_ORDER=3, main :915-922)."""
     def enum(field, v):
@@ -74,7 +74,7 @@ def make_config(K=50, p=3, x0=0.0, y0=0.0, L=1.0, gamma=1.4,
                     enum("bc", bc), int(bool(av)), int(bool(limiter)),
                     enum("precision", precision), float(av_cav), float(av_ksensor),
                     float(eps_rho), float(eps_p), int(elems_per_block), int(xcd_remap),
-                    enum("scheme", scheme))
+                    enum("scheme", scheme), int(mfma))
 
 
 def sd_config(K=50, p=3, precision="fp64", **kw):

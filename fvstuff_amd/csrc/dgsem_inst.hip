@@ -71,12 +71,12 @@ static hipError_t set_lds(K kern, size_t lds, size_t& done) {
 #define DG_CAT2(a, b) a##b
 #define DG_CAT(a, b) DG_CAT2(a, b)
 
-template <typename Real, bool FULL, int VOL>
+template <typename Real, bool FULL, int VOL, bool MFMA = false>
 static hipError_t launch_one(const KArgs& a, int grid, int block, size_t lds, hipStream_t s) {
     static size_t done = 0;
-    hipError_t e = set_lds(stage_kernel<DG_P, Real, FULL, VOL>, lds, done);
+    hipError_t e = set_lds(stage_kernel<DG_P, Real, FULL, VOL, MFMA>, lds, done);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((stage_kernel<DG_P, Real, FULL, VOL>), dim3(grid), dim3(block), lds, s, a);
+    hipLaunchKernelGGL((stage_kernel<DG_P, Real, FULL, VOL, MFMA>), dim3(grid), dim3(block), lds, s, a);
     return hipGetLastError();
 }
 
@@ -89,8 +89,20 @@ static hipError_t launch_vol(const KArgs& a, int grid, int block, size_t lds, hi
     }
 }
 
-hipError_t DG_CAT(launch_stage_p, DG_P)(const KArgs& a, bool fp32, bool full, int grid, int block, size_t lds,
-                                        hipStream_t s) {
+hipError_t DG_CAT(launch_stage_p, DG_P)(const KArgs& a, bool fp32, bool full, bool mfma, int grid, int block,
+                                        size_t lds, hipStream_t s) {
+#if DG_P == 3
+    // the FP32 shock-capturing kernels with the AV on the matrix cores
+    if (mfma && fp32 && full && a.vol != VOL_SD) {
+        switch (a.vol) {
+            case VOL_RANOCHA: return launch_one<float, true, VOL_RANOCHA, true>(a, grid, block, lds, s);
+            case VOL_CHANDRASHEKAR: return launch_one<float, true, VOL_CHANDRASHEKAR, true>(a, grid, block, lds, s);
+            default: return launch_one<float, true, VOL_COLLOCATION, true>(a, grid, block, lds, s);
+        }
+    }
+#else
+    if (mfma) return hipErrorInvalidValue;
+#endif
     // the spectral-difference sibling has one (lean) kernel per precision
     if (a.vol == VOL_SD)
         return fp32 ? launch_one<float, false, VOL_SD>(a, grid, block, lds, s)

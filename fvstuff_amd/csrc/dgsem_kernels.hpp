@@ -502,10 +502,15 @@ template <int P, int VOL, bool FULL> struct Occ<P, double, VOL, FULL> {
 // batched kernel, whose large workgroups are LDS-limited, runs faster so; the
 // single-solve kernels run slower, measured).
 // U0_EARLY: U^n is loaded with the stage input at the start (see below).
+// MFMA_AV (p = 3, FP32, shock-capturing kernels): the artificial viscosity's
+// tensor-product contraction on the matrix cores instead of in the line
+// phase (see the AV pre-pass below).
 template <int P, typename Real, bool FULL, int VOL, bool PIN = (sizeof(Real) == 8), bool UP_LDS = true,
-          bool U0_EARLY = !FULL, bool PAIR_OK = true>
+          bool U0_EARLY = !FULL, bool PAIR_OK = true, bool MFMA_AV = false>
 __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int tile, bool lead) {
     static_assert(UP_LDS || !FULL, "the limiter needs the stage values in LDS");
+    static_assert(!MFMA_AV || (P == 3 && sizeof(Real) == 4 && FULL && VOL != VOL_SD),
+                  "the MFMA form of the AV is the p = 3 FP32 shock-capturing kernel's");
     constexpr bool SD = VOL == VOL_SD;
     static_assert(UP_LDS || !SD, "the SD line reads the stage input from LDS");
     constexpr int N = P + 1, N2 = N * N;
@@ -546,6 +551,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     Real* sDu = (Real*)(smem + a.off_av);           // [E][N2] AV: du/dx
     Real* sDv = sDu + E * N2;                       // [E][N2] AV: dv/dy
     Real* sNu = sDv + E * N2;                       // [E]     AV: element viscosity
+    Real* sUc = sNu + E;                            // [E][4][N2] MFMA_AV: conserved nodes, then the AV
     unsigned long long* sLamE = (unsigned long long*)(smem + a.off_lam);   // [E] max wave speed (bits)
     unsigned long long* sThE = sLamE + E;           // [E] max sensor (bits of the double)
     double* sEl = (double*)(smem + a.off_el);       // per-element scratch
@@ -761,6 +767,13 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         Real* p = sW + s * 5;
         p[0] = (Real)0.5 * w.r; p[1] = (Real)0.5 * w.u; p[2] = (Real)0.5 * w.v;
         p[3] = (Real)0.5 * w.p; p[4] = (Real)0.5 * w.z;
+        if (MFMA_AV && a.av) {
+            // the conserved variables the AV acts on, from the prims as the
+            // line phase's VALU form computes them
+            Real* u = sUc + e * 4 * N2 + nd;
+            u[0] = w.r; u[N2] = w.r * w.u; u[2 * N2] = w.r * w.v;
+            u[3 * N2] = w.p * ph.inv_gm1 + (Real)0.5 * w.r * (w.u * w.u + w.v * w.v);
+        }
         if ((FULL || SD) && need_lam) {
             const unsigned long long sp =
                 dbits(SD ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri));
@@ -816,6 +829,45 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         if (tid < E)
             sNu[tid] = (Real)a.av_cav * (Real)bitsd(sThE[tid]) * len_p * (Real)bitsd(sLamE[tid]);
         __syncthreads();
+        if (MFMA_AV) {
+            // AV_q = -nu jac^2 (U_q Lav^T + Lav U_q) on the 4 x 4 node array
+            // U_q[jn][in] of each element and variable, i.e. Lav along the
+            // x-lines plus Lav along the y-lines (:501-521 in one operator,
+            // Lav = W^-1 D^T W D), as 16 blocks of 4 x 4 x 1 MFMA per
+            // instruction: block b = (element e, variable q), lane 4b + c.
+            // The instruction forms D[r][c] += A[r] B[c] from the A values of
+            // the block's lanes 4b + r and the B value of lane 4b + c (checked
+            // on gfx950).  x-term, step m: A[r] = U_q[r][m], B[c] = Lav[c][m];
+            // y-term: A[r] = Lav[r][m], B[c] = U_q[m][c].  Both give the
+            // output at node (jn = r, in = c), written back over U_q (each wave
+            // owns the 8 elements of its lines, and its reads precede its
+            // writes).  Elements past the tile hold the stand-in state.
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            const Real* Lav = D + 2 * N * N + 2 * N;
+            const int l = tid & 63, c = l & 3, q = (l >> 2) & 3;
+            float lav[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) lav[m] = Lav[c * 4 + m];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int e = 8 * (tid >> 6) + 4 * h + (l >> 4);
+                const bool ok = e < E;
+                Real* U = sUc + (ok ? e : 0) * 4 * N2 + q * N2;
+                float ux[4], uy[4];
+#pragma unroll
+                for (int m = 0; m < 4; m++) { ux[m] = U[c * 4 + m]; uy[m] = U[m * 4 + c]; }
+                f4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int m = 0; m < 4; m++) d = __builtin_amdgcn_mfma_f32_4x4x1f32(ux[m], lav[m], d, 0, 0, 0);
+#pragma unroll
+                for (int m = 0; m < 4; m++) d = __builtin_amdgcn_mfma_f32_4x4x1f32(lav[m], uy[m], d, 0, 0, 0);
+                const Real f = -sNu[ok ? e : 0] * jac * jac;
+                if (ok) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) U[r * 4 + c] = f * d[r];
+                }
+            }
+        }
     }
 
     // ── B: line phase ──
@@ -897,7 +949,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
 
         // ── artificial viscosity along the line: -jac W^-1 D^T W (nu jac D Uc)
         //    = -nu jac^2 Lav Uc, Lav = W^-1 D^T W D precomputed (one pass) ──
-        if (FULL && !SD && a.av) {
+        if (FULL && !SD && !MFMA_AV && a.av) {
             const Real f = -sNu[le] * jac * jac;
             const Real* Lav = D + 2 * N * N + 2 * N;
             Real uc[N][4];      // conserved variables of the line's nodes (u, v as rotated)
@@ -955,13 +1007,18 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     __syncthreads();
 
     // ── C: node phase ──
+    // R of node nd of element e, variable q (with the MFMA form's AV)
+    auto rhs = [&](int e, int nd, int q) -> Real {
+        const Real r = sR[e * 4 * N2 + q * N2 + nd];
+        if (MFMA_AV && a.av) return r + sUc[e * 4 * N2 + q * N2 + nd];
+        return r;
+    };
     if (a.mode == MODE_RHS) {
         for (int s = tid; s < nE * N2; s += nthr) {
             const int e = s / N2, nd = s - e * N2;
             const size_t o = (size_t)(el0 + e) * N2 + nd;
-            const Real* r = sR + e * 4 * N2 + nd;
 #pragma unroll
-            for (int q = 0; q < 4; q++) a.out[q * S + o] = (double)r[q * N2];
+            for (int q = 0; q < 4; q++) a.out[q * S + o] = (double)rhs(e, nd, q);
         }
         if ((FULL || SD) && a.lam_el != nullptr && tid < nE) a.lam_el[el0 + tid] = bitsd(sLamE[tid]);
         return;
@@ -978,17 +1035,16 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     // sanitize_cons, the SD repair
     auto stage_value = [&](int j, int s, double (&Us)[4]) {
         const int e = s / N2, nd = s - e * N2;
-        const Real* r = sR + e * 4 * N2 + nd;
         const double* up = sU + e * 4 * N2 + nd;
         double Ui[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) Ui[q] = UP_LDS ? up[q * N2] : Upl[j][q];
         if (a.ca != 0.0) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) Us[q] = a.ca * U0l[j][q] + a.cb * (Ui[q] + dt * (double)r[q * N2]);
+            for (int q = 0; q < 4; q++) Us[q] = a.ca * U0l[j][q] + a.cb * (Ui[q] + dt * (double)rhs(e, nd, q));
         } else {
 #pragma unroll
-            for (int q = 0; q < 4; q++) Us[q] = Ui[q] + dt * (double)r[q * N2];
+            for (int q = 0; q < 4; q++) Us[q] = Ui[q] + dt * (double)rhs(e, nd, q);
         }
         Cons64::sanitize_cons(Us, gs);
         if (SD && a.limiter) Cons64::repair(Us, gs, a.eps_rho, a.eps_p);
@@ -1088,10 +1144,11 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     }
 }
 
-template <int P, typename Real, bool FULL, int VOL>
+template <int P, typename Real, bool FULL, int VOL, bool MFMA_AV = false>
 __global__ __launch_bounds__(BLOCK, (Occ<P, Real, VOL, FULL>::waves)) void stage_kernel(KArgs a) {
     const StepCtl sc{a.slot_cur, a.slot_next, a.check_blowup};
-    stage_tile<P, Real, FULL, VOL>(a, sc, tile_of_block(blockIdx.x, gridDim.x, a.xcd), blockIdx.x == 0);
+    stage_tile<P, Real, FULL, VOL, (sizeof(Real) == 8), true, !FULL, true, MFMA_AV>(
+        a, sc, tile_of_block(blockIdx.x, gridDim.x, a.xcd), blockIdx.x == 0);
 }
 
 // Batched stage: many independent solves (own K, p, domain, dt) in one
@@ -1151,14 +1208,16 @@ inline int align_up(int x, int a) { return (x + a - 1) / a * a; }
 
 // `stage`: the FP64 input state is staged in LDS (sU) for the update phase;
 // `lim`: the limiter's per-element scratch.
-inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, bool lam, bool lim) {
+// `mfma`: the MFMA form of the AV (its conserved-node planes, [E][4][N2]).
+inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, bool lam, bool lim,
+                            bool mfma = false) {
     const int N2 = N * N;
     LdsLayout L;
     int off = align_up(E * N2 * 5 * real_bytes, 16);
     L.off_U = off;
     if (stage) off += align_up(E * 4 * N2 * 8, 16);
     L.off_av = off;
-    if (av) off += align_up((2 * E * N2 + E) * real_bytes, 16);
+    if (av) off += align_up((2 * E * N2 + E + (mfma ? 4 * E * N2 : 0)) * real_bytes, 16);
     L.off_lam = off;
     if (lam || av) off += align_up(2 * E * 8, 16);
     L.off_el = off;

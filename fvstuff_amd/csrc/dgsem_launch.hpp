@@ -5,8 +5,9 @@
 
 namespace dg {
 
-// Launches stage_kernel<P, Real, FULL>.  Returns the launch's hipError_t.
-typedef hipError_t (*StageLauncher)(const KArgs& a, bool fp32, bool full, int grid, int block,
+// Launches stage_kernel<P, Real, FULL> (mfma: its p = 3 FP32 shock-capturing
+// form with the AV on the matrix cores).  Returns the launch's hipError_t.
+typedef hipError_t (*StageLauncher)(const KArgs& a, bool fp32, bool full, bool mfma, int grid, int block,
                                     size_t lds, hipStream_t s);
 // Limiter on E-element tiles in place.
 typedef hipError_t (*LimitLauncher)(double* Q, const double* w64, int NEL, int E, double g, double eps_rho,
@@ -16,7 +17,7 @@ typedef hipError_t (*SensorLauncher)(const double* Q, double* sensor, const doub
                                      double g, double ksensor, hipStream_t s);
 
 #define DG_DECLARE_P(P)                                                                              \
-    hipError_t launch_stage_p##P(const KArgs&, bool, bool, int, int, size_t, hipStream_t);          \
+    hipError_t launch_stage_p##P(const KArgs&, bool, bool, bool, int, int, size_t, hipStream_t);    \
     hipError_t launch_limit_p##P(double*, const double*, int, int, double, double, double, hipStream_t); \
     hipError_t launch_sensor_p##P(const double*, double*, const double*, int, double, double, double, hipStream_t);
 DG_DECLARE_P(1)

@@ -32,12 +32,13 @@ struct dg_ctx {
     double* d_xi = nullptr;              // LGL node coordinates (IC kernel)
     int part_blocks = 0;
     int E = 0, xcd = 0;
+    bool mfma = false;                   // AV contraction on the matrix cores (p = 3, FP32, av)
     int last_steps = 0;                  // steps taken by the last dg_advance
     // graph replay of the fused loop: two steps per replay, captured on `cap`
     hipStream_t cap = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipGraphExec_t gexec = nullptr;
-    struct { const double* Q; double cfl, t_end; int E, xcd; } gkey = {nullptr, 0, 0, 0, 0};
+    struct { const double* Q; double cfl, t_end; int E, xcd; bool mfma; } gkey = {nullptr, 0, 0, 0, 0, false};
     bool use_graph = true;
     char err[512] = {0};
 };
@@ -549,6 +550,11 @@ static int default_E(int N, int NEL) {
     return E;
 }
 
+// The MFMA form of the AV exists for the p = 3 FP32 shock-capturing kernels.
+static bool mfma_capable(const dg_config& f) {
+    return f.p == 3 && f.precision == DG_MIXED && f.av && f.scheme == DG_SCHEME_DGSEM;
+}
+
 // ── stage launch ───────────────────────────────────────────────────────────
 struct StageCall {
     int mode, stage;
@@ -559,15 +565,16 @@ struct StageCall {
     int check_blowup, slot_cur, slot_next;
 };
 
-static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd, hipStream_t stream) {
+static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd, hipStream_t stream, int mfma = -1) {
     const dg_config& f = c->cfg;
     const bool fp32 = f.precision == DG_MIXED;
     const bool sd = f.scheme == DG_SCHEME_SD;
     const bool lam = s.mode == MODE_RHS && s.lam_el != nullptr;
     const bool lim = s.mode == MODE_STAGE && f.limiter && !sd;   // Zhang-Shu scratch
     const bool full = !sd && (f.av || lim || lam);
+    const bool mf = (mfma < 0 ? c->mfma : mfma != 0) && mfma_capable(f);
     // the SD line reads the FP64 stage input from LDS in both modes
-    LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, s.mode == MODE_STAGE || sd, f.av != 0, lam, lim);
+    LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, s.mode == MODE_STAGE || sd, f.av != 0, lam, lim, mf);
     KArgs a;
     memset(&a, 0, sizeof(a));
     a.Up = s.Up; a.U0 = s.U0; a.out = s.out; a.lam_el = s.lam_el;
@@ -584,7 +591,7 @@ static int launch_stage(dg_ctx* c, const StageCall& s, int E, int xcd, hipStream
     set_derived(a);
     const int grid = (c->NEL + E - 1) / E;
     const int block = std::min(BLOCK, (E * 2 * c->N + 63) / 64 * 64);
-    HIPCK(c, stage_launcher(f.p)(a, fp32, full, grid, block, (size_t)L.total, stream));
+    HIPCK(c, stage_launcher(f.p)(a, fp32, full, mf, grid, block, (size_t)L.total, stream));
     return DG_OK;
 }
 
@@ -643,6 +650,7 @@ void dg_config_default(dg_config* f) {
     f->av_cav = 0.25; f->av_ksensor = 0.1; f->eps_rho = 1e-6; f->eps_p = 1e-6;
     f->elems_per_block = 0; f->xcd_remap = -1;
     f->scheme = DG_SCHEME_DGSEM;
+    f->mfma = -1;
 }
 
 void dg_ic_default(dg_ic_params* ic, int kind) {
@@ -711,6 +719,7 @@ int dg_create(const dg_config* cfg, void* stream, dg_ctx** out) {
         cfg->volume_flux < 0 || cfg->volume_flux > 2 || cfg->surface_flux < 0 || cfg->surface_flux > 1 ||
         cfg->bc < 0 || cfg->bc > 1 || cfg->precision < 0 || cfg->precision > 1 ||
         cfg->scheme < 0 || cfg->scheme > 1 || (cfg->scheme == DG_SCHEME_SD && cfg->av) ||
+        cfg->mfma < -1 || cfg->mfma > 1 ||
         (long long)cfg->K * cfg->K > (1LL << 30) / ((cfg->p + 1) * (cfg->p + 1)))
         return DG_ERR_INVALID_ARG;
     dg_ctx* c = new dg_ctx();
@@ -726,6 +735,7 @@ int dg_create(const dg_config* cfg, void* stream, dg_ctx** out) {
     const int Emax = max_elems_per_block(c->N);
     c->E = cfg->elems_per_block > 0 ? std::min(cfg->elems_per_block, Emax) : default_E(c->N, c->NEL);
     c->xcd = cfg->xcd_remap >= 0 ? (cfg->xcd_remap != 0) : 1;
+    c->mfma = cfg->mfma > 0 && mfma_capable(*cfg);
 
     // tables: D[N2], w[N], winv[N], Dv[N2], Lav[N2], Isf[(N+1)N], Dsd[N(N+1)].
     // Dv is the volume matrix, D scaled by the volume term's factor -2/h
@@ -884,7 +894,7 @@ static int advance_impl(dg_ctx* c, double* Q, int nsteps, double cfl, double t_e
     if (graph && nsteps >= 3) {
         // steps 1, 2, ... replay a graph of two steps (odd slot parity first)
         const bool same = c->gexec && c->gkey.Q == Q && c->gkey.cfl == cfl && c->gkey.t_end == t_end &&
-                          c->gkey.E == E && c->gkey.xcd == xcd;
+                          c->gkey.E == E && c->gkey.xcd == xcd && c->gkey.mfma == c->mfma;
         if (!same) {
             if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
             hipGraph_t g;
@@ -900,7 +910,7 @@ static int advance_impl(dg_ctx* c, double* Q, int nsteps, double cfl, double t_e
             hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
             hipGraphDestroy(g);
             HIPCK(c, ei);
-            c->gkey = {Q, cfl, t_end, E, xcd};
+            c->gkey = {Q, cfl, t_end, E, xcd, c->mfma};
         }
         HIPCK(c, hipEventRecord(c->ev_in, c->stream));
         HIPCK(c, hipStreamWaitEvent(c->cap, c->ev_in, 0));
@@ -1099,12 +1109,18 @@ int dg_autotune(dg_ctx* c, int* elems_per_block, int* xcd_remap) {
     hipEventCreate(&e1);
     float best = 1e30f;
     int bestE = c->E, bestX = c->xcd;
+    bool bestM = c->mfma;
+    // the two forms of the AV contraction (VALU in the line phase, MFMA) when
+    // the configuration leaves the choice to the tuner
+    const int nm = (c->cfg.mfma < 0 && mfma_capable(c->cfg)) ? 2 : 1;
     // every candidate starts from the same smooth state and takes a few small
     // steps (CFL 0.1 is stable for every degree), so numerics never interfere;
     // a candidate's time is the better of two 4-step runs (schedules within
     // 1 % of each other are otherwise ranked by noise)
+    for (int mi = 0; mi < nm && rc == DG_OK; mi++)
     for (int x = 0; x < 2 && rc == DG_OK; x++)
         for (int E : Es) {
+            if (nm == 2) c->mfma = mi != 0;
             double t = 0.0;
             if ((rc = dg_init_state(c, &ic, Q))) break;
             rc = advance_impl(c, Q, 1, 0.1, 0.0, &t, nullptr, E, x, false);   // warm-up (LDS attribute, caches)
@@ -1120,11 +1136,12 @@ int dg_autotune(dg_ctx* c, int* elems_per_block, int* xcd_remap) {
                 cand = std::min(cand, ms);
             }
             if (rc) break;
-            if (cand < best) { best = cand; bestE = E; bestX = x; }
+            if (cand < best) { best = cand; bestE = E; bestX = x; bestM = c->mfma; }
         }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     hipFree(Q);
+    c->mfma = bestM;
     if (rc) return rc;
     c->E = bestE;
     c->xcd = bestX;

@@ -77,7 +77,8 @@ def test_scheme_validation_before_any_device_call():
     ctx = C.c_void_p()
     for bad in (capi.sd_config(K=4, p=3, av=True), capi.make_config(K=4, p=3, scheme=2),
                 capi.sd_config(K=4, p=8), capi.make_config(K=0, p=3), capi.make_config(K=4, p=0),
-                capi.make_config(K=4, p=3, L=0.0), capi.make_config(K=4, p=3, gamma=1.0)):
+                capi.make_config(K=4, p=3, L=0.0), capi.make_config(K=4, p=3, gamma=1.0),
+                capi.make_config(K=4, p=3, mfma=2), capi.make_config(K=4, p=3, mfma=-2)):
         assert lib.dg_create(C.byref(bad), None, C.byref(ctx)) == capi.DG_ERR_INVALID_ARG
         assert not ctx.value
     arr = (capi.DGConfig * 1)(capi.sd_config(K=4, p=3, limiter=False))

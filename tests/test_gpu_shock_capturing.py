@@ -178,3 +178,59 @@ def test_mixed_kh_fused_loop(dgsem, p, K):
     assert err < 1e-6
     for q in (0, 3):
         assert abs(T1[q] - T0[q]) <= 1e-7 * abs(T0[q])
+
+
+MFMA_CASES = [
+    ("kh-p3-E8", kh_sc(8, 3, elems_per_block=8, mfma=1), "kh"),
+    # workgroups of two and four waves (each wave contracts its own elements)
+    ("kh-p3-E16", kh_sc(8, 3, elems_per_block=16, mfma=1), "kh"),
+    ("kh-p3-E32", kh_sc(8, 3, elems_per_block=32, mfma=1), "kh"),
+    # partial workgroups and a partial last tile
+    ("kh-p3-E3-ragged", kh_sc(7, 3, elems_per_block=3, mfma=1), "kh"),
+    ("kh-p3-E4", kh_sc(6, 3, elems_per_block=4, mfma=1), "kh"),
+    ("sod-walls-p3", make_config(K=12, p=3, precision="mixed", mfma=1), "sod"),
+]
+
+
+@pytest.mark.parametrize("name,cfg,ic", MFMA_CASES, ids=[c[0] for c in MFMA_CASES])
+def test_mfma_av_rhs(dgsem, name, cfg, ic):
+    """The AV contraction on the matrix cores (4x4x1 FP32 MFMA blocks, one
+    per element and variable) against the oracle, and against the VALU form
+    of the same kernel (the same FP32 operator, another summation order)."""
+    o = Oracle(with_precision(cfg, "fp64"))
+    Q = o.init_state(make_ic("kh", noise_amp=0.3)) if ic == "kh" else sod_moving_state(cfg.K, cfg.p)
+    R_ref, lam_ref = o.compute_rhs(Q)
+    R, lam = dgsem.Solver(cfg).compute_rhs(gpu(Q))
+    err = rel_l2(cpu(R), R_ref, w2(cfg))
+    cfg_valu = type(cfg).from_buffer_copy(cfg)
+    cfg_valu.mfma = 0
+    R_valu, _ = dgsem.Solver(cfg_valu).compute_rhs(gpu(Q))
+    d = rel_l2(cpu(R), cpu(R_valu), w2(cfg))
+    print(f"{name}: MFMA AV RHS rel L2 vs oracle {err:.2e}, vs the VALU form {d:.2e}")
+    assert err < rhs_bar(cfg.p)
+    assert d < 1e-6
+    np.testing.assert_allclose(cpu(lam), lam_ref, rtol=1e-6)
+
+
+@pytest.mark.parametrize("case", ["blast", "kh"])
+def test_mfma_av_fused_loop(dgsem, case):
+    """20 fused steps with the MFMA form of the AV: the blast (limiter
+    firing) and the KH workload, within 1e-6 of the oracle."""
+    if case == "blast":
+        ic, cfl = make_ic("sod", rho_hi=1.0, p_hi=1e6), 0.4
+        cfg = make_config(K=10, p=3, precision="mixed", mfma=1)
+    else:
+        ic, cfl = make_ic("kh"), 0.5
+        cfg = kh_sc(16, 3, mfma=1)
+    o = Oracle(with_precision(cfg, "fp64"))
+    Q_ref = o.init_state(ic)
+    Q0 = Q_ref.copy()
+    rc, t_ref, _ = o.advance(Q_ref, 20, cfl)
+    assert rc == 0
+    s = dgsem.Solver(cfg)
+    Q = gpu(Q0)
+    t, _ = s.advance(Q, 20, cfl)
+    err = rel_l2(cpu(Q), Q_ref, w2(cfg))
+    print(f"{case}: MFMA AV state rel L2 vs oracle after 20 steps {err:.2e}")
+    assert t == pytest.approx(t_ref, rel=1e-6)
+    assert err < 1e-6
