@@ -76,6 +76,9 @@ def parse():
     ap.add_argument("--mfma", type=int, default=-1, choices=[-1, 0, 1],
                     help="kh: the AV contraction on the matrix cores (1), on the VALU (0), or tuned (-1)")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--persistent", type=int, default=-1, choices=[-1, 0, 1],
+                    help="batched: persistent stage launches (1), one workgroup per tile (0), tuned (-1)")
+    ap.add_argument("--lanes", type=int, default=0, help="batched: lanes per workgroup (0 = tuned)")
     ap.add_argument("--seed", type=int, default=2024)
     return ap.parse_args()
 
@@ -290,7 +293,13 @@ def bench_batched(args):
     cfgs, ics, cfls = ensemble(n=args.instances, seed=args.seed, precision=form)
     b = D.Batch(cfgs)
     Qs = b.init_states(ics)
-    lanes = b.autotune(Qs, cfls) if not args.no_autotune else None
+    tuned = not (args.no_autotune or args.lanes or args.persistent >= 0)
+    if tuned:
+        b.autotune(Qs, cfls)
+    else:
+        b.set_options(use_graph=True, lanes=args.lanes)
+        b.set_persistent(max(0, args.persistent))
+    _, lanes, persistent = b.options()
     res = {}
     for mode in ("batched", "each"):
         Qs = b.init_states(ics, Qs)
@@ -306,6 +315,7 @@ def bench_batched(args):
         log(f"[bench-batched] {mode}: {res[mode]}")
     print(json.dumps({"workload": f"{args.instances} vortex instances (seed {args.seed}), {form}",
                       "unit": "DOF*stage/s", "dof": b.dofs, "steps": args.steps, "lanes_per_workgroup": lanes,
+                      "persistent": persistent, "tuned": tuned,
                       "batched": res["batched"], "one_by_one": res["each"],
                       "speedup": res["batched"]["value"] / res["each"]["value"]}))
 

@@ -70,6 +70,7 @@ struct KArgs {
     const double* w64;   // FP64 GLL weights (limiter)
     DevState* st;
     int P, K, NEL, E, xcd;
+    int zero;            // always 0: see batch_stage_persistent
     int mode, stage, vol, surf, bc, av, limiter, slot_cur, slot_next;
     int check_blowup;    // stage 1: the wave speed in slot_cur came from a step of this call
     double h, gamma, ca, cb, cfl, t_end, av_cav, av_k, eps_rho, eps_p;
@@ -507,7 +508,7 @@ template <int P, int VOL, bool FULL> struct Occ<P, double, VOL, FULL> {
 // phase (see the AV pre-pass below).
 template <int P, typename Real, bool FULL, int VOL, bool PIN = (sizeof(Real) == 8), bool UP_LDS = true,
           bool U0_EARLY = !FULL, bool PAIR_OK = true, bool MFMA_AV = false>
-__device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int tile, bool lead) {
+__device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int tile, bool lead, int tid) {
     static_assert(UP_LDS || !FULL, "the limiter needs the stage values in LDS");
     static_assert(!MFMA_AV || (P == 3 && sizeof(Real) == 4 && FULL && VOL != VOL_SD),
                   "the MFMA form of the AV is the p = 3 FP32 shock-capturing kernel's");
@@ -527,7 +528,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     // before it writes)
     double* s_dt = (double*)smem;
     int* s_go = (int*)(smem + 8);
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int nthr = blockDim.x;
     const int E = a.E;
     const int el0 = tile * E;
     const int nE = min(E, a.NEL - el0);
@@ -1148,7 +1149,7 @@ template <int P, typename Real, bool FULL, int VOL, bool MFMA_AV = false>
 __global__ __launch_bounds__(BLOCK, (Occ<P, Real, VOL, FULL>::waves)) void stage_kernel(KArgs a) {
     const StepCtl sc{a.slot_cur, a.slot_next, a.check_blowup};
     stage_tile<P, Real, FULL, VOL, (sizeof(Real) == 8), true, !FULL, true, MFMA_AV>(
-        a, sc, tile_of_block(blockIdx.x, gridDim.x, a.xcd), blockIdx.x == 0);
+        a, sc, tile_of_block(blockIdx.x, gridDim.x, a.xcd), blockIdx.x == 0, (int)threadIdx.x);
 }
 
 // Batched stage: many independent solves (own K, p, domain, dt) in one
@@ -1173,6 +1174,21 @@ template <typename Real> struct BatchOcc {
     static constexpr bool pair = sizeof(Real) == 8;
 };
 
+// One tile of the batched stage: the instance's degree selects the body.
+template <typename Real, bool FULL, int VOL>
+__device__ __forceinline__ void batch_tile(const KArgs& a, const StepCtl sc, int tile, bool lead, int tid) {
+    constexpr bool UL = BatchOcc<Real>::up_lds, UE = BatchOcc<Real>::u0_early, PR = BatchOcc<Real>::pair;
+    switch (a.P) {
+        case 1: stage_tile<1, Real, FULL, VOL, true, UL, UE, PR>(a, sc, tile, lead, tid); break;
+        case 2: stage_tile<2, Real, FULL, VOL, true, UL, UE, PR>(a, sc, tile, lead, tid); break;
+        case 3: stage_tile<3, Real, FULL, VOL, true, UL, UE, PR>(a, sc, tile, lead, tid); break;
+        case 4: stage_tile<4, Real, FULL, VOL, true, UL, UE, PR>(a, sc, tile, lead, tid); break;
+        case 5: stage_tile<5, Real, FULL, VOL, true, UL, UE, PR>(a, sc, tile, lead, tid); break;
+        case 6: stage_tile<6, Real, FULL, VOL, true, UL, UE, PR>(a, sc, tile, lead, tid); break;
+        default: stage_tile<7, Real, FULL, VOL, true, UL, UE, PR>(a, sc, tile, lead, tid); break;
+    }
+}
+
 template <typename Real, bool FULL, int VOL>
 __global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_kernel(
     const KArgs* __restrict__ args, const int2* __restrict__ tiles, int ntiles, int stage, StepCtl sc) {
@@ -1180,22 +1196,24 @@ __global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_ke
     if (t >= ntiles) return;
     const int2 tl = tiles[t];
     const KArgs& a = args[(size_t)tl.x * 3 + (stage - 1)];
-    const bool lead = tl.y == 0;
-    switch (a.P) {
-        case 1: stage_tile<1, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
-        case 2: stage_tile<2, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
-        case 3: stage_tile<3, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
-        case 4: stage_tile<4, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
-        case 5: stage_tile<5, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
-        case 6: stage_tile<6, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
-        default: stage_tile<7, Real, FULL, VOL, true, BatchOcc<Real>::up_lds,
-                                   BatchOcc<Real>::u0_early, BatchOcc<Real>::pair>(a, sc, tl.y, lead); break;
+    batch_tile<Real, FULL, VOL>(a, sc, tl.y, tl.y == 0, (int)threadIdx.x);
+}
+
+// Persistent form of the batched stage: a grid of as many workgroups as are
+// resident on the chip at once walks the tile list (heaviest first) with a
+// grid stride.  The thread index is taken through the tile's argument block
+// (tid ^ a.zero, a.zero = 0): otherwise the compiler hoists the index
+// arithmetic of all seven degree bodies out of the loop and keeps it live
+// across it (256 registers and spills).  The barrier at the end of an
+// iteration keeps the next tile's LDS writes behind this tile's reads.
+template <typename Real, bool FULL, int VOL>
+__global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_persistent(
+    const KArgs* __restrict__ args, const int2* __restrict__ tiles, int ntiles, int stage, StepCtl sc) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 tl = tiles[t];
+        const KArgs& a = args[(size_t)tl.x * 3 + (stage - 1)];
+        batch_tile<Real, FULL, VOL>(a, sc, tl.y, tl.y == 0, (int)threadIdx.x ^ a.zero);
+        __syncthreads();
     }
 }
 

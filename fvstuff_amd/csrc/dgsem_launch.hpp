@@ -30,11 +30,15 @@ DG_DECLARE_P(7)
 #undef DG_DECLARE_P
 
 // Batched stage launchers (dgsem_batch.hip, one object per precision): one
-// workgroup of `lanes` threads per tile.
+// workgroup of `lanes` threads per tile (grid = 0), or the persistent form on
+// `grid` workgroups.
 hipError_t launch_batch_stage_float(const KArgs* args, const int2* tiles, int ntiles, int stage, StepCtl sc,
-                                    int vol, int lanes, size_t lds, hipStream_t s);
+                                    int vol, int lanes, size_t lds, int grid, hipStream_t s);
 hipError_t launch_batch_stage_double(const KArgs* args, const int2* tiles, int ntiles, int stage, StepCtl sc,
-                                     int vol, int lanes, size_t lds, hipStream_t s);
+                                     int vol, int lanes, size_t lds, int grid, hipStream_t s);
+// Resident workgroups per CU of the persistent batched kernel.
+hipError_t batch_persistent_per_cu_float(int vol, int lanes, size_t lds, int* per_cu);
+hipError_t batch_persistent_per_cu_double(int vol, int lanes, size_t lds, int* per_cu);
 
 constexpr int PMAX_GPU = 7;
 constexpr int BATCH_BLOCK = 64;   // default lanes per workgroup of the batched stage kernel

@@ -1168,6 +1168,8 @@ struct dg_batch {
     int ntiles = 0, lanes = BATCH_BLOCK, tiles_cap = 0;
     size_t lds = 0;
     bool use_graph = true;
+    bool persistent = false;            // persistent form of the stage launches
+    int pgrid = 0;                      // its grid: resident workgroups (0 = unavailable)
     hipGraphExec_t gexec = nullptr;
     char err[512] = {0};
 };
@@ -1190,10 +1192,11 @@ static int bfail(dg_batch* b, int code, const char* fmt, ...) {
 
 static hipError_t batch_launch_stage(dg_batch* b, int stage, StepCtl sc) {
     const bool fp32 = b->common.precision == DG_MIXED;
+    const int grid = b->persistent ? b->pgrid : 0;
     return fp32 ? launch_batch_stage_float(b->d_args, b->d_tiles, b->ntiles, stage, sc, b->common.volume_flux,
-                                           b->lanes, b->lds, b->cap)
+                                           b->lanes, b->lds, grid, b->cap)
                 : launch_batch_stage_double(b->d_args, b->d_tiles, b->ntiles, stage, sc, b->common.volume_flux,
-                                            b->lanes, b->lds, b->cap);
+                                            b->lanes, b->lds, grid, b->cap);
 }
 
 static hipError_t batch_step(dg_batch* b, StepCtl sc) {
@@ -1267,6 +1270,17 @@ static hipError_t batch_tiles(dg_batch* b) {
     }
     b->ntiles = (int)tiles.size();
     b->lds = lds;
+    // the persistent form's grid: the workgroups resident on the chip at once
+    {
+        const bool fp32 = b->common.precision == DG_MIXED;
+        int per_cu = 0, ncu = 0, dev = 0;
+        hipError_t e = fp32 ? batch_persistent_per_cu_float(b->common.volume_flux, b->lanes, lds, &per_cu)
+                            : batch_persistent_per_cu_double(b->common.volume_flux, b->lanes, lds, &per_cu);
+        if (e == hipSuccess) e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        b->pgrid = std::min(b->ntiles, std::max(0, per_cu) * ncu);
+    }
     hipError_t e = hipMemcpyAsync(b->d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice,
                                   b->stream);
     if (e != hipSuccess) return e;
@@ -1379,6 +1393,15 @@ int dg_batch_set_options(dg_batch* b, int use_graph, int lanes) {
     return DG_OK;
 }
 
+int dg_batch_set_persistent(dg_batch* b, int persistent) {
+    if (!b) return DG_ERR_INVALID_ARG;
+    if (persistent && b->pgrid < 1)
+        return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_set_persistent: no resident workgroup at this size");
+    if ((persistent != 0) != b->persistent && b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    b->persistent = persistent != 0;
+    return DG_OK;
+}
+
 int dg_batch_advance(dg_batch* b, double* const* Qs, int nsteps, const double* cfl, double t_end, double* t_inout,
                      int* status_out) {
     if (!b || !Qs || !cfl || nsteps < 0) return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_advance: bad argument");
@@ -1444,9 +1467,14 @@ int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* la
     hipEventCreate(&e1);
     float best = 1e30f;
     int best_lanes = b->lanes;
+    bool best_pers = b->persistent;
     int rc = DG_OK;
-    for (int l : cands) {
+    // each size in both forms: one workgroup per tile, and persistent
+    for (int cand = 0; cand < 6; cand++) {
+        const int l = cands[cand >> 1];
         if ((rc = dg_batch_set_options(b, b->use_graph, l))) break;
+        if ((cand & 1) && b->pgrid < 1) continue;
+        if ((rc = dg_batch_set_persistent(b, cand & 1))) break;
         // note: the states are advanced by 4 steps per candidate at cfl[i]
         if ((rc = dg_batch_advance(b, Qs, 1, cfl, 0.0, nullptr, nullptr))) break;   // warm-up + graph
         hipEventRecord(e0, b->stream);
@@ -1455,13 +1483,22 @@ int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* la
         hipEventSynchronize(e1);
         float ms = 0.f;
         hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best) { best = ms; best_lanes = l; }
+        if (ms < best) { best = ms; best_lanes = l; best_pers = b->persistent; }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     int rc2 = dg_batch_set_options(b, b->use_graph, best_lanes);
+    if (!rc2) rc2 = dg_batch_set_persistent(b, best_pers);
     if (lanes) *lanes = b->lanes;
     return rc ? rc : rc2;
+}
+
+int dg_batch_get_options(const dg_batch* b, int* use_graph, int* lanes, int* persistent) {
+    if (!b) return DG_ERR_INVALID_ARG;
+    if (use_graph) *use_graph = b->use_graph;
+    if (lanes) *lanes = b->lanes;
+    if (persistent) *persistent = b->persistent;
+    return DG_OK;
 }
 
 }  // extern "C"

@@ -63,6 +63,8 @@ SIGNATURES = {
     "dg_batch_advance_each": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, _dp, C.c_double, _dp, C.POINTER(C.c_int)]),
     "dg_batch_set_options": (C.c_int, [_vp, C.c_int, C.c_int]),
     "dg_batch_autotune": (C.c_int, [_vp, C.POINTER(_vp), _dp, C.POINTER(C.c_int)]),
+    "dg_batch_set_persistent": (C.c_int, [_vp, C.c_int]),
+    "dg_batch_get_options": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }
 
 
@@ -345,7 +347,19 @@ class Batch:
     def set_options(self, use_graph=True, lanes=0):
         self._check(self.lib.dg_batch_set_options(self._b, int(bool(use_graph)), int(lanes)))
 
+    def set_persistent(self, persistent):
+        """Persistent form of the stage launches (resident workgroups walk the tiles)."""
+        self._check(self.lib.dg_batch_set_persistent(self._b, int(bool(persistent))))
+
+    def options(self):
+        """(use_graph, lanes, persistent) currently in use."""
+        g, l, p = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.lib.dg_batch_get_options(self._b, C.byref(g), C.byref(l), C.byref(p)))
+        return bool(g.value), l.value, bool(p.value)
+
     def autotune(self, Qs, cfl):
+        """Time the workgroup sizes in both launch forms and keep the fastest;
+        returns the lanes per workgroup (see options() for the form)."""
         cfl = [float(cfl)] * self.n if isinstance(cfl, (int, float)) else [float(c) for c in cfl]
         g = C.c_int()
         self._check(self.lib.dg_batch_autotune(self._b, self._ptrs(Qs), (C.c_double * self.n)(*cfl), C.byref(g)))

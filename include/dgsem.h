@@ -293,8 +293,15 @@ int         dg_batch_advance_each(dg_batch* b, double* const* Qs, int nsteps, co
 /* Tuning: use_graph (0/1); lanes per workgroup, 64 / 128 / 256 (0 = 64):
  * instance i's tiles hold max(1, lanes / (2(p_i+1))) elements. */
 int         dg_batch_set_options(dg_batch* b, int use_graph, int lanes);
-/* Time the three workgroup sizes on the given states (they are advanced by
- * a few steps) and keep the fastest; writes it to *lanes if non-NULL. */
+/* Persistent form of the stage launches (0 = one workgroup per tile, the
+ * default; 1 = as many workgroups as are resident on the chip walk the tile
+ * list with a grid stride).  Results are identical either way. */
+int         dg_batch_set_persistent(dg_batch* b, int persistent);
+/* Current options (any pointer may be NULL). */
+int         dg_batch_get_options(const dg_batch* b, int* use_graph, int* lanes, int* persistent);
+/* Time the three workgroup sizes, each with one workgroup per tile and in
+ * the persistent form, on the given states (they are advanced by a few steps)
+ * and keep the fastest; writes the size to *lanes if non-NULL. */
 int         dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* lanes);
 
 #ifdef __cplusplus
