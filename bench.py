@@ -62,6 +62,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the error report")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="vortex: skip the short samples of the other workloads (KH 1024^2 / 2048^2 p=3, "
+                         "256 batched instances)")
+    ap.add_argument("--extra-steps", type=int, default=30,
+                    help="timed steps of each short sample of the other workloads")
     ap.add_argument("--schedule", default="",
                     help="E,X: elements per workgroup, XCD remap (implies --no-autotune)")
     ap.add_argument("--cpu-steps", type=int, default=15,
@@ -163,33 +168,39 @@ def cpu_baseline(cfg, ic, steps, cfl, what="vortex", single_steps=1, keep_state=
     return (out, kept) if keep_state else out
 
 
-def bench_kh(args):
-    """Kelvin-Helmholtz step shear layer on periodic [-1,1]^2 to t_end in the
-    mixed form (FP32 RHS, FP64 accumulation), with the FP64 form run to the
-    same time for the error; entropy and total-energy drift reported."""
+def run_kh(K, p, forms, cfl=0.5, seed=2024, t_end=None, steps=None, schedule="", autotune=True, mfma=-1,
+           shock=True, t_compare=0.01, tag="bench-kh"):
+    """Kelvin-Helmholtz step shear layer on periodic [-1,1]^2 (rho 2 / 1,
+    u +-0.5, p 2.5, v = 0.1 sin(2 pi x) + seeded noise 1e-3) with the
+    reference's shock capturing (Chandrashekar + HLLC, Ducros AV, Zhang-Shu
+    limiter), in each form: either to t_end or for a fixed number of steps,
+    timed with HIP events (state resident in HBM).  Beside the rate: mass,
+    energy and entropy drift, and the mixed form against the FP64 form at the
+    early time t_compare (both forms advanced there from the IC, untimed;
+    later the roll-up is chaotic and the two forms drift apart)."""
     import torch
     from fvstuff_amd import dgsem as D
     from fvstuff_amd.capi import kh_config, make_ic
-    ic = make_ic("kh", seed=args.seed, noise_amp=1e-3)
-    K, p = args.K, args.p
+    ic = make_ic("kh", seed=seed, noise_amp=1e-3)
     N = p + 1
     dof = 4 * K * K * N * N
-    out = {}
-    Qs = {}
-    for form in args.forms.split(","):
-        # the reference's own configuration (Chandrashekar + HLLC, Ducros AV,
-        # Zhang-Shu limiter): without its shock capturing the unresolved step
-        # layer loses positivity at t ~ 0.15-0.25 (Ranocha or Chandrashekar,
-        # Rusanov or HLLC); with it the layer rolls up to t = 1.5
-        sc = not args.no_shock_capturing
+    out, early = {}, {}
+    for form in forms:
+        # without the shock capturing the unresolved step layer loses
+        # positivity at t ~ 0.15-0.25 (Ranocha or Chandrashekar, Rusanov or
+        # HLLC); with it the layer rolls up to t = 1.5
         sched = {}
-        if args.schedule:
-            E_, X_ = (int(v) for v in args.schedule.split(","))
+        if schedule:
+            E_, X_ = (int(v) for v in schedule.split(","))
             sched = dict(elems_per_block=E_, xcd_remap=X_)
-        s = D.Solver(kh_config(K=K, p=p, precision=form, av=sc, limiter=sc,
-                               volume_flux="chandrashekar", surface_flux="hllc", mfma=args.mfma, **sched))
-        if not (args.no_autotune or args.schedule):
-            s.autotune()
+        s = D.Solver(kh_config(K=K, p=p, precision=form, av=shock, limiter=shock,
+                               volume_flux="chandrashekar", surface_flux="hllc", mfma=mfma, **sched))
+        tuned = s.autotune() if (autotune and not schedule) else None
+        if t_compare:
+            Qc = s.init_state(ic)
+            t_c = early["fp64"][1] if (form == "mixed" and "fp64" in early) else t_compare
+            tc, _ = s.advance_to(Qc, t_c, cfl)
+            early[form] = (s, tc, Qc)
         Q = s.init_state(ic)
         T0 = s.totals(Q)
         torch.cuda.synchronize()
@@ -197,34 +208,53 @@ def bench_kh(args):
         e0.record()
         last = [time.perf_counter()]
 
-        def progress(t, steps):
+        def progress(t, n):
             if time.perf_counter() - last[0] > 10.0:
                 last[0] = time.perf_counter()
-                log(f"[bench-kh] {form}: t={t:.4f} steps={steps}")
+                log(f"[{tag}] {form}: t={t:.4f} steps={n}")
 
-        t, steps = s.advance_to(Q, args.t_end, args.cfl, chunk=512, progress=progress)
+        if steps:
+            t, _ = s.advance(Q, steps, cfl)
+            taken, launched = s.steps_taken(), steps
+        else:
+            t, taken = s.advance_to(Q, t_end, cfl, chunk=512, progress=progress)
+            launched = s.last_launched
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1)
         T1 = s.totals(Q)
-        rate = dof * 3 * steps / (ms * 1e-3)
+        rate = dof * 3 * taken / (ms * 1e-3)
         # the rate counts the steps taken; the timed region also holds the
         # no-op steps launched past t_end (each still streams the state once)
-        out[form] = {"value": rate, "steps": steps, "steps_launched": s.last_launched, "t": t, "ms": ms,
+        out[form] = {"value": rate, "steps": taken, "steps_launched": launched, "t": t, "ms": ms,
+                     "us_per_launch": ms * 1e3 / (3 * launched), "schedule": tuned,
                      "hbm_GBs_algorithmic": rate * (64.0 / 3.0) / 1e9,
                      "hbm_frac": rate * (64.0 / 3.0) / 1e9 / HBM_PEAK_GBS,
                      "energy_drift_rel": (T1[3] - T0[3]) / abs(T0[3]),
                      "mass_drift_rel": (T1[0] - T0[0]) / abs(T0[0]),
                      "entropy_change_rel": (T1[4] - T0[4]) / abs(T0[4])}
-        Qs[form] = Q
-        log(f"[bench-kh] {form}: {out[form]}")
-    if "fp64" in Qs and "mixed" in Qs:
-        nrm = s.error_norms(Qs["mixed"], Qs["fp64"])
-        out["mixed_vs_fp64_rel_l2"] = nrm["rel_l2"]
-        out["mixed_vs_fp64_rel_linf"] = nrm["rel_linf"]
-    print(json.dumps({"workload": f"Kelvin-Helmholtz {K}x{K} p={p} to t={args.t_end}, CFL {args.cfl}, "
-                                  f"Chandrashekar + HLLC, AV + limiter {'off' if args.no_shock_capturing else 'on'}",
-                      "unit": "DOF*stage/s", "dof": dof, **out}))
+        log(f"[{tag}] {form}: {out[form]}")
+        del Q
+    if "fp64" in out and "mixed" in out:
+        out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    if "fp64" in early and "mixed" in early:
+        s64, t64, Q64 = early["fp64"]
+        nrm = s64.error_norms(early["mixed"][2], Q64)
+        out["early_compare"] = {"t": t64, "mixed_vs_fp64_rel_l2": nrm["rel_l2"],
+                                "mixed_vs_fp64_rel_linf": nrm["rel_linf"]}
+    return {"workload": f"Kelvin-Helmholtz {K}x{K} p={p}, "
+                        + (f"{steps} steps" if steps else f"to t={t_end}")
+                        + f", CFL {cfl}, Chandrashekar + HLLC, AV + limiter {'on' if shock else 'off'}",
+            "unit": "DOF*stage/s", "dof": dof, **out}
+
+
+def bench_kh(args):
+    """The Kelvin-Helmholtz workload to --t-end (run_kh); the mixed form
+    against the FP64 form at t = 0.01 and at the end time."""
+    res = run_kh(args.K, args.p, args.forms.split(","), cfl=args.cfl, seed=args.seed, t_end=args.t_end,
+                 schedule=args.schedule, autotune=not args.no_autotune, mfma=args.mfma,
+                 shock=not args.no_shock_capturing)
+    print(json.dumps(res))
 
 
 def bench_sd(args):
@@ -282,42 +312,46 @@ def bench_sd(args):
                       "unit": "DOF*stage/s", "dof": dof, **out}))
 
 
-def bench_batched(args):
+def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, autotune=True):
     """Seeded ensemble of isentropic-vortex instances (K ~ U{16..128},
     p ~ U{3,5,7}, strength ~ U[1,5], Mach ~ U[0.1,0.8]) stepped in one
-    launch per stage (one workgroup per tile) vs instance by instance."""
+    launch per stage for all instances vs instance by instance (wall clock
+    around each, after a synchronisation)."""
     import torch
     from fvstuff_amd import dgsem as D
     from fvstuff_amd.capi import ensemble
-    form = args.forms.split(",")[-1]
-    cfgs, ics, cfls = ensemble(n=args.instances, seed=args.seed, precision=form)
+    cfgs, ics, cfls = ensemble(n=instances, seed=seed, precision=form)
     b = D.Batch(cfgs)
     Qs = b.init_states(ics)
-    tuned = not (args.no_autotune or args.lanes or args.persistent >= 0)
+    tuned = autotune and not (lanes or persistent >= 0)
     if tuned:
         b.autotune(Qs, cfls)
     else:
-        b.set_options(use_graph=True, lanes=args.lanes)
-        b.set_persistent(max(0, args.persistent))
+        b.set_options(use_graph=True, lanes=lanes)
+        b.set_persistent(max(0, persistent))
     _, lanes, persistent = b.options()
     res = {}
     for mode in ("batched", "each"):
         Qs = b.init_states(ics, Qs)
         fn = b.advance if mode == "batched" else b.advance_each
-        fn(Qs, args.warmup, cfls)
+        fn(Qs, warmup, cfls)
         Qs = b.init_states(ics, Qs)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        fn(Qs, args.steps, cfls)
+        fn(Qs, steps, cfls)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        res[mode] = {"value": b.dofs * 3 * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps}
-        log(f"[bench-batched] {mode}: {res[mode]}")
-    print(json.dumps({"workload": f"{args.instances} vortex instances (seed {args.seed}), {form}",
-                      "unit": "DOF*stage/s", "dof": b.dofs, "steps": args.steps, "lanes_per_workgroup": lanes,
-                      "persistent": persistent, "tuned": tuned,
-                      "batched": res["batched"], "one_by_one": res["each"],
-                      "speedup": res["batched"]["value"] / res["each"]["value"]}))
+        res[mode] = {"value": b.dofs * 3 * steps / wall, "ms_per_step": wall * 1e3 / steps}
+        log(f"[bench-batched] {form} {mode}: {res[mode]}")
+    return {"workload": f"{instances} vortex instances (seed {seed}), {form}", "unit": "DOF*stage/s",
+            "dof": b.dofs, "steps": steps, "lanes_per_workgroup": lanes, "persistent": persistent, "tuned": tuned,
+            "batched": res["batched"], "one_by_one": res["each"],
+            "speedup": res["batched"]["value"] / res["each"]["value"]}
+
+
+def bench_batched(args):
+    print(json.dumps(run_batched(args.instances, args.seed, args.forms.split(",")[-1], args.steps, args.warmup,
+                                 lanes=args.lanes, persistent=args.persistent, autotune=not args.no_autotune)))
 
 
 def bench_copy(args):
@@ -421,6 +455,19 @@ def main():
             acc["vs_oracle"] = {"steps": k, "t_oracle": to, **vs}
             log(f"[bench] vs oracle after {k} steps: {acc['vs_oracle']}")
 
+    # short, bounded samples of the other workloads (after the headline's
+    # solvers are released): KH 1024^2 and 2048^2 p = 3 in both forms, and
+    # the 256-instance batch; rates over --extra-steps timed steps
+    extra = None
+    if not args.no_extra:
+        del sols
+        extra = {}
+        for Kx in (1024, 2048):
+            extra[f"kelvin_helmholtz_{Kx}_p3"] = run_kh(Kx, 3, ["fp64", "mixed"], steps=args.extra_steps,
+                                                        tag=f"bench-kh{Kx}")
+        for form in ("mixed", "fp64"):
+            extra[f"batched_256_vortex_{form}"] = run_batched(256, 2024, form, args.extra_steps, 3)
+
     src_hash = kernel_sources_hash()
     traffic = PMC_TRAFFIC.get((K, p, main_form))
     traffic_ok = traffic is not None and traffic[2] == src_hash
@@ -451,6 +498,7 @@ def main():
                      "kernel_sources": src_hash,
                      "bytes_per_dof_stage": b_alg, "avg_launch_ms": ms_launch},
         "cpu_baseline": cpu,
+        "other_workloads": extra,
     }
     print(json.dumps(out))
 
