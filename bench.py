@@ -84,6 +84,8 @@ def parse():
     ap.add_argument("--persistent", type=int, default=-1, choices=[-1, 0, 1],
                     help="batched: persistent stage launches (1), one workgroup per tile (0), tuned (-1)")
     ap.add_argument("--lanes", type=int, default=0, help="batched: lanes per workgroup (0 = tuned)")
+    ap.add_argument("--xcd-bins", type=int, default=-1, choices=[-1, 0, 1],
+                    help="batched: tile list in per-XCD bins (1), heaviest first (0), tuned (-1)")
     ap.add_argument("--seed", type=int, default=2024)
     return ap.parse_args()
 
@@ -108,6 +110,40 @@ def time_form(D, torch, cfg, ic, steps, warmup, cfl, autotune):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
     return s, Q, ms, t, tuned
+
+
+def hbm_reference(n, reps=20):
+    """One step's HBM byte pattern of the fused stages moved by torch's own
+    elementwise kernels, for reference: a copy (stage 1 reads U^n and writes
+    U1) and two two-input combinations (stages 2 and 3 read two vectors and
+    write one), on FP64 vectors of n values (the state's size), HIP events
+    around `reps` such steps: 64 n bytes per step, as the stage kernels'
+    algorithmic traffic."""
+    import torch
+    U = torch.rand(n, dtype=torch.float64, device="cuda")
+    U1 = torch.empty_like(U)
+    U2 = torch.empty_like(U)
+
+    def step():
+        U1.copy_(U)
+        torch.add(U, U1, alpha=0.25, out=U2)
+        torch.add(U, U2, alpha=2.0 / 3.0, out=U1)
+
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = 64.0 * n / (ms * 1e-3) / 1e9
+    del U, U1, U2
+    return {"ms_per_step": ms, "GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
+            "method": "torch copy_ + 2 x add(out=) on FP64 vectors of the state's size, 64 B per DOF per step"}
 
 
 def cpu_model():
@@ -312,7 +348,7 @@ def bench_sd(args):
                       "unit": "DOF*stage/s", "dof": dof, **out}))
 
 
-def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, autotune=True):
+def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, xcd_bins=-1, autotune=True):
     """Seeded ensemble of isentropic-vortex instances (K ~ U{16..128},
     p ~ U{3,5,7}, strength ~ U[1,5], Mach ~ U[0.1,0.8]) stepped in one
     launch per stage for all instances vs instance by instance (wall clock
@@ -323,13 +359,14 @@ def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, au
     cfgs, ics, cfls = ensemble(n=instances, seed=seed, precision=form)
     b = D.Batch(cfgs)
     Qs = b.init_states(ics)
-    tuned = autotune and not (lanes or persistent >= 0)
+    tuned = autotune and not (lanes or persistent >= 0 or xcd_bins >= 0)
     if tuned:
         b.autotune(Qs, cfls)
     else:
         b.set_options(use_graph=True, lanes=lanes)
+        b.set_xcd_bins(max(0, xcd_bins))
         b.set_persistent(max(0, persistent))
-    _, lanes, persistent = b.options()
+    _, lanes, persistent, xcd_bins = b.options()
     res = {}
     for mode in ("batched", "each"):
         Qs = b.init_states(ics, Qs)
@@ -344,14 +381,16 @@ def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, au
         res[mode] = {"value": b.dofs * 3 * steps / wall, "ms_per_step": wall * 1e3 / steps}
         log(f"[bench-batched] {form} {mode}: {res[mode]}")
     return {"workload": f"{instances} vortex instances (seed {seed}), {form}", "unit": "DOF*stage/s",
-            "dof": b.dofs, "steps": steps, "lanes_per_workgroup": lanes, "persistent": persistent, "tuned": tuned,
+            "dof": b.dofs, "steps": steps, "lanes_per_workgroup": lanes, "persistent": persistent,
+            "xcd_bins": xcd_bins, "tuned": tuned,
             "batched": res["batched"], "one_by_one": res["each"],
             "speedup": res["batched"]["value"] / res["each"]["value"]}
 
 
 def bench_batched(args):
     print(json.dumps(run_batched(args.instances, args.seed, args.forms.split(",")[-1], args.steps, args.warmup,
-                                 lanes=args.lanes, persistent=args.persistent, autotune=not args.no_autotune)))
+                                 lanes=args.lanes, persistent=args.persistent, xcd_bins=args.xcd_bins,
+                                 autotune=not args.no_autotune)))
 
 
 def bench_copy(args):
@@ -468,6 +507,10 @@ def main():
         for form in ("mixed", "fp64"):
             extra[f"batched_256_vortex_{form}"] = run_batched(256, 2024, form, args.extra_steps, 3)
 
+    ref = hbm_reference(dof)
+    ref["stage_kernels_over_reference"] = ref["ms_per_step"] / (res[main_form]["ms"] / args.steps)
+    log(f"[bench] HBM reference: {ref}")
+
     src_hash = kernel_sources_hash()
     traffic = PMC_TRAFFIC.get((K, p, main_form))
     traffic_ok = traffic is not None and traffic[2] == src_hash
@@ -496,7 +539,8 @@ def main():
                                         f"not measured for kernel sources {src_hash}"),
                      "kernel": f"stage_kernel<{p},{'float' if main_form == 'mixed' else 'double'},lean,Ranocha>",
                      "kernel_sources": src_hash,
-                     "bytes_per_dof_stage": b_alg, "avg_launch_ms": ms_launch},
+                     "bytes_per_dof_stage": b_alg, "avg_launch_ms": ms_launch,
+                     "hbm_reference": ref},
         "cpu_baseline": cpu,
         "other_workloads": extra,
     }

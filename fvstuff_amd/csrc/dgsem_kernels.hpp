@@ -1189,12 +1189,15 @@ __device__ __forceinline__ void batch_tile(const KArgs& a, const StepCtl sc, int
     }
 }
 
+// One workgroup per list entry; entries with instance -1 pad the XCD-binned
+// list (see batch_tiles) and do nothing.
 template <typename Real, bool FULL, int VOL>
 __global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_kernel(
     const KArgs* __restrict__ args, const int2* __restrict__ tiles, int ntiles, int stage, StepCtl sc) {
     const int t = blockIdx.x;
     if (t >= ntiles) return;
     const int2 tl = tiles[t];
+    if (tl.x < 0) return;
     const KArgs& a = args[(size_t)tl.x * 3 + (stage - 1)];
     batch_tile<Real, FULL, VOL>(a, sc, tl.y, tl.y == 0, (int)threadIdx.x);
 }
@@ -1206,11 +1209,14 @@ __global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_ke
 // arithmetic of all seven degree bodies out of the loop and keeps it live
 // across it (256 registers and spills).  The barrier at the end of an
 // iteration keeps the next tile's LDS writes behind this tile's reads.
+// (With a grid that is a multiple of 8, a block walks only list positions
+// of its own XCD's bin.)
 template <typename Real, bool FULL, int VOL>
 __global__ __launch_bounds__(BLOCK, (BatchOcc<Real>::waves)) void batch_stage_persistent(
     const KArgs* __restrict__ args, const int2* __restrict__ tiles, int ntiles, int stage, StepCtl sc) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 tl = tiles[t];
+        if (tl.x < 0) continue;
         const KArgs& a = args[(size_t)tl.x * 3 + (stage - 1)];
         batch_tile<Real, FULL, VOL>(a, sc, tl.y, tl.y == 0, (int)threadIdx.x ^ a.zero);
         __syncthreads();

@@ -615,6 +615,7 @@ __global__ void batch_lambda_kernel(const KArgs* args, const int2* tiles, int nt
     __shared__ double sred[BLOCK / 64];
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 tl = tiles[t];
+        if (tl.x < 0) continue;   // padding of the XCD-binned list
         const KArgs& a = args[(size_t)tl.x * 3];
         const int N2 = (a.P + 1) * (a.P + 1);
         const size_t S = (size_t)a.NEL * N2;
@@ -1169,6 +1170,7 @@ struct dg_batch {
     size_t lds = 0;
     bool use_graph = true;
     bool persistent = false;            // persistent form of the stage launches
+    bool xcd_bins = false;              // tile list in per-XCD bins (batch_tiles)
     int pgrid = 0;                      // its grid: resident workgroups (0 = unavailable)
     hipGraphExec_t gexec = nullptr;
     char err[512] = {0};
@@ -1240,27 +1242,61 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
 }
 
 // (Re)build the tile list for the current lanes per workgroup: instance i
-// is cut into tiles of batch_elems_per_block(N_i, lanes) elements, heaviest
-// tiles first (cost of a tile ~ E (p+1)^3: pairs per line x lines); the LDS
-// size is the largest instance's.
+// is cut into tiles of batch_elems_per_block(N_i, lanes) elements, cost of a
+// tile ~ E (p+1)^3 (pairs per line x lines); the LDS size is the largest
+// instance's.  Two orders:
+//  - global (xcd_bins off): every tile, heaviest first (the longest tasks
+//    start first, the short ones fill the tail);
+//  - XCD bins: the instances bin-packed over the 8 XCDs by cost (largest
+//    first, each into the least loaded bin), each bin's tiles heaviest first
+//    with an instance's tiles contiguous, and the bins interleaved (position
+//    8k + x holds bin x's k-th tile, shorter bins padded with empty entries,
+//    instance -1).  The hardware deals workgroups round-robin over the XCDs,
+//    so XCD x runs bin x in order: the tiles of an instance, which read each
+//    other's face nodes, share an L2 and run close in time.
 static hipError_t batch_tiles(dg_batch* b) {
-    std::vector<std::pair<double, int2>> tl;
+    std::vector<int2> tiles;
     size_t lds = 0;
+    std::vector<double> tile_cost(b->n);
+    std::vector<int> ntl(b->n);
     for (int i = 0; i < b->n; i++) {
         dg_ctx* c = b->ctx[i];
         const int E = batch_elems_per_block(c->N, b->lanes);
-        const int nt = (c->NEL + E - 1) / E;
-        const double cost = (double)E * c->N * c->N * c->N;
-        for (int t = 0; t < nt; t++) tl.push_back({cost, make_int2(i, t)});
+        ntl[i] = (c->NEL + E - 1) / E;
+        tile_cost[i] = (double)E * c->N * c->N * c->N;
         const bool fp32 = b->common.precision == DG_MIXED;
         LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, !fp32, false, false, false);
         lds = std::max(lds, (size_t)L.total);
     }
-    std::stable_sort(tl.begin(), tl.end(), [](const std::pair<double, int2>& x, const std::pair<double, int2>& y) {
-        return x.first > y.first;
-    });
-    std::vector<int2> tiles(tl.size());
-    for (size_t k = 0; k < tl.size(); k++) tiles[k] = tl[k].second;
+    // instances by tile cost, heaviest first (stable: ties keep their order)
+    std::vector<int> order(b->n);
+    for (int i = 0; i < b->n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return tile_cost[x] > tile_cost[y]; });
+    if (!b->xcd_bins) {
+        for (int i : order)
+            for (int t = 0; t < ntl[i]; t++) tiles.push_back(make_int2(i, t));
+    } else {
+        std::vector<int> by_total(b->n);
+        for (int i = 0; i < b->n; i++) by_total[i] = i;
+        std::stable_sort(by_total.begin(), by_total.end(), [&](int x, int y) {
+            return tile_cost[x] * ntl[x] > tile_cost[y] * ntl[y];
+        });
+        double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        std::vector<int> bin_of(b->n);
+        for (int i : by_total) {
+            const int x = (int)(std::min_element(load, load + 8) - load);
+            bin_of[i] = x;
+            load[x] += tile_cost[i] * ntl[i];
+        }
+        std::vector<int2> bins[8];
+        for (int i : order)
+            for (int t = 0; t < ntl[i]; t++) bins[bin_of[i]].push_back(make_int2(i, t));
+        size_t len = 0;
+        for (int x = 0; x < 8; x++) len = std::max(len, bins[x].size());
+        tiles.assign(8 * len, make_int2(-1, 0));
+        for (int x = 0; x < 8; x++)
+            for (size_t k = 0; k < bins[x].size(); k++) tiles[8 * k + x] = bins[x][k];
+    }
     if ((int)tiles.size() > b->tiles_cap) {
         hipFree(b->d_tiles);
         b->d_tiles = nullptr;
@@ -1270,7 +1306,8 @@ static hipError_t batch_tiles(dg_batch* b) {
     }
     b->ntiles = (int)tiles.size();
     b->lds = lds;
-    // the persistent form's grid: the workgroups resident on the chip at once
+    // the persistent form's grid: the workgroups resident on the chip at
+    // once, a multiple of 8 (a block then walks its own XCD's bin)
     {
         const bool fp32 = b->common.precision == DG_MIXED;
         int per_cu = 0, ncu = 0, dev = 0;
@@ -1279,7 +1316,8 @@ static hipError_t batch_tiles(dg_batch* b) {
         if (e == hipSuccess) e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (e != hipSuccess) return e;
-        b->pgrid = std::min(b->ntiles, std::max(0, per_cu) * ncu);
+        int g = std::min(b->ntiles, std::max(0, per_cu) * ncu);
+        b->pgrid = g >= 8 ? g / 8 * 8 : g;
     }
     hipError_t e = hipMemcpyAsync(b->d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice,
                                   b->stream);
@@ -1393,6 +1431,8 @@ int dg_batch_set_options(dg_batch* b, int use_graph, int lanes) {
     return DG_OK;
 }
 
+int dg_batch_set_xcd_bins(dg_batch* b, int xcd_bins);
+
 int dg_batch_set_persistent(dg_batch* b, int persistent) {
     if (!b) return DG_ERR_INVALID_ARG;
     if (persistent && b->pgrid < 1)
@@ -1469,10 +1509,13 @@ int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* la
     int best_lanes = b->lanes;
     bool best_pers = b->persistent;
     int rc = DG_OK;
-    // each size in both forms: one workgroup per tile, and persistent
-    for (int cand = 0; cand < 6; cand++) {
-        const int l = cands[cand >> 1];
+    bool best_bins = b->xcd_bins;
+    // each size with both list orders, each in both launch forms (one
+    // workgroup per tile, persistent)
+    for (int cand = 0; cand < 12; cand++) {
+        const int l = cands[cand >> 2];
         if ((rc = dg_batch_set_options(b, b->use_graph, l))) break;
+        if ((rc = dg_batch_set_xcd_bins(b, (cand >> 1) & 1))) break;
         if ((cand & 1) && b->pgrid < 1) continue;
         if ((rc = dg_batch_set_persistent(b, cand & 1))) break;
         // note: the states are advanced by 4 steps per candidate at cfl[i]
@@ -1483,21 +1526,32 @@ int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* la
         hipEventSynchronize(e1);
         float ms = 0.f;
         hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best) { best = ms; best_lanes = l; best_pers = b->persistent; }
+        if (ms < best) { best = ms; best_lanes = l; best_pers = b->persistent; best_bins = b->xcd_bins; }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     int rc2 = dg_batch_set_options(b, b->use_graph, best_lanes);
+    if (!rc2) rc2 = dg_batch_set_xcd_bins(b, best_bins);
     if (!rc2) rc2 = dg_batch_set_persistent(b, best_pers);
     if (lanes) *lanes = b->lanes;
     return rc ? rc : rc2;
 }
 
-int dg_batch_get_options(const dg_batch* b, int* use_graph, int* lanes, int* persistent) {
+int dg_batch_get_options(const dg_batch* b, int* use_graph, int* lanes, int* persistent, int* xcd_bins) {
     if (!b) return DG_ERR_INVALID_ARG;
     if (use_graph) *use_graph = b->use_graph;
     if (lanes) *lanes = b->lanes;
     if (persistent) *persistent = b->persistent;
+    if (xcd_bins) *xcd_bins = b->xcd_bins;
+    return DG_OK;
+}
+
+int dg_batch_set_xcd_bins(dg_batch* b, int xcd_bins) {
+    if (!b) return DG_ERR_INVALID_ARG;
+    if ((xcd_bins != 0) == b->xcd_bins) return DG_OK;
+    b->xcd_bins = xcd_bins != 0;
+    BHIPCK(b, batch_tiles(b));
+    if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }   // the list is baked into the graph
     return DG_OK;
 }
 

@@ -64,7 +64,9 @@ SIGNATURES = {
     "dg_batch_set_options": (C.c_int, [_vp, C.c_int, C.c_int]),
     "dg_batch_autotune": (C.c_int, [_vp, C.POINTER(_vp), _dp, C.POINTER(C.c_int)]),
     "dg_batch_set_persistent": (C.c_int, [_vp, C.c_int]),
-    "dg_batch_get_options": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "dg_batch_set_xcd_bins": (C.c_int, [_vp, C.c_int]),
+    "dg_batch_get_options": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]),
 }
 
 
@@ -351,11 +353,15 @@ class Batch:
         """Persistent form of the stage launches (resident workgroups walk the tiles)."""
         self._check(self.lib.dg_batch_set_persistent(self._b, int(bool(persistent))))
 
+    def set_xcd_bins(self, xcd_bins):
+        """Tile list in per-XCD bins of instances (True) or all tiles heaviest first."""
+        self._check(self.lib.dg_batch_set_xcd_bins(self._b, int(bool(xcd_bins))))
+
     def options(self):
-        """(use_graph, lanes, persistent) currently in use."""
-        g, l, p = C.c_int(), C.c_int(), C.c_int()
-        self._check(self.lib.dg_batch_get_options(self._b, C.byref(g), C.byref(l), C.byref(p)))
-        return bool(g.value), l.value, bool(p.value)
+        """(use_graph, lanes, persistent, xcd_bins) currently in use."""
+        g, l, p, x = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self._check(self.lib.dg_batch_get_options(self._b, C.byref(g), C.byref(l), C.byref(p), C.byref(x)))
+        return bool(g.value), l.value, bool(p.value), bool(x.value)
 
     def autotune(self, Qs, cfl):
         """Time the workgroup sizes in both launch forms and keep the fastest;

@@ -297,11 +297,16 @@ int         dg_batch_set_options(dg_batch* b, int use_graph, int lanes);
  * default; 1 = as many workgroups as are resident on the chip walk the tile
  * list with a grid stride).  Results are identical either way. */
 int         dg_batch_set_persistent(dg_batch* b, int persistent);
+/* Order of the tile list (0 = all tiles heaviest first, the default; 1 =
+ * instances bin-packed by cost over the 8 XCDs, each bin heaviest first,
+ * the bins interleaved so that each XCD runs one bin and the tiles of an
+ * instance share an L2).  Results are identical either way. */
+int         dg_batch_set_xcd_bins(dg_batch* b, int xcd_bins);
 /* Current options (any pointer may be NULL). */
-int         dg_batch_get_options(const dg_batch* b, int* use_graph, int* lanes, int* persistent);
-/* Time the three workgroup sizes, each with one workgroup per tile and in
- * the persistent form, on the given states (they are advanced by a few steps)
- * and keep the fastest; writes the size to *lanes if non-NULL. */
+int         dg_batch_get_options(const dg_batch* b, int* use_graph, int* lanes, int* persistent, int* xcd_bins);
+/* Time the three workgroup sizes, each with both list orders and both
+ * launch forms, on the given states (they are advanced by a few steps) and
+ * keep the fastest; writes the size to *lanes if non-NULL. */
 int         dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* lanes);
 
 #ifdef __cplusplus

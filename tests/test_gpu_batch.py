@@ -145,23 +145,27 @@ def test_batch_blowup_found_when_trailing_steps_are_noops(D):
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_batch_persistent_matches(D, precision):
     """The persistent form of the stage launches (resident workgroups walk
-    the tile list) gives the same bits as one workgroup per tile, at every
-    workgroup size, with and without graph replay."""
+    the tile list) and the XCD-binned list order (padded with empty entries)
+    give the same bits as one workgroup per tile in the global order, at
+    every workgroup size, with and without graph replay."""
     cfgs, ics, cfls = ensemble(n=9, seed=4, precision=precision, k_range=(4, 20), degrees=(1, 2, 3, 4, 5, 6, 7))
     b = D.Batch(cfgs)
     ref = None
     for lanes in (64, 128, 256):
         for persistent in (False, True):
-            for use_graph in (True, False):
-                b.set_options(use_graph=use_graph, lanes=lanes)
-                b.set_persistent(persistent)
-                assert b.options() == (use_graph, lanes, persistent)
-                Qs = b.init_states(ics)
-                ts = b.advance(Qs, 7, cfls)
-                got = (ts, host(Qs))
-                if ref is None:
-                    ref = got
-                    continue
-                assert got[0] == ref[0], (lanes, persistent, use_graph)
-                for a, c in zip(got[1], ref[1]):
-                    assert np.array_equal(a, c), (lanes, persistent, use_graph)
+            for bins in (False, True):
+                for use_graph in (True, False):
+                    b.set_options(use_graph=use_graph, lanes=lanes)
+                    b.set_xcd_bins(bins)
+                    b.set_persistent(persistent)
+                    assert b.options() == (use_graph, lanes, persistent, bins)
+                    Qs = b.init_states(ics)
+                    ts = b.advance(Qs, 7, cfls)
+                    got = (ts, host(Qs))
+                    if ref is None:
+                        ref = got
+                        continue
+                    key = (lanes, persistent, bins, use_graph)
+                    assert got[0] == ref[0], key
+                    for a, c in zip(got[1], ref[1]):
+                        assert np.array_equal(a, c), key
