@@ -310,9 +310,13 @@ def bench_sd(args):
     cfl = 0.1 / p
     ic = make_ic("sod")
     out, Qs = {}, {}
+    sched = {}
+    if args.schedule:
+        E_, X_ = (int(v) for v in args.schedule.split(","))
+        sched = dict(elems_per_block=E_, xcd_remap=X_)
     for form in args.forms.split(","):
-        s = D.Solver(sd_config(K=K, p=p, precision=form))
-        tuned = None if args.no_autotune else s.autotune()
+        s = D.Solver(sd_config(K=K, p=p, precision=form, **sched))
+        tuned = None if (args.no_autotune or args.schedule) else s.autotune()
         Q = s.init_state(ic)
         if args.warmup:
             s.advance(Q, args.warmup, cfl)
