@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 # (bytes, profile that lists the runs and the scaling, KERNEL_SOURCES hash of
 # the build that was profiled).  A figure whose hash differs from the current
 # sources' is not reported (the kernel changed since it was measured).
-PMC_TRAFFIC = {(512, 7, "mixed"): (1.437e9, "profiles/stage_kernel_p7_vortex512.md", "unmeasured")}
+PMC_TRAFFIC = {(512, 7, "mixed"): (1.437e9, "profiles/stage_kernel_p7_vortex512.md", "72af120452059468")}
 FP64_PEAK_TFLOPS = 78.6
 # the sources the stage kernels are compiled from
 KERNEL_SOURCES = ("fvstuff_amd/csrc/dgsem_kernels.hpp", "fvstuff_amd/csrc/dgsem_physics.hpp",
@@ -78,6 +78,10 @@ def parse():
     ap.add_argument("--t-end", type=float, default=1.5)
     ap.add_argument("--no-shock-capturing", action="store_true",
                     help="kh: switch off the Ducros AV and the Zhang-Shu limiter")
+    ap.add_argument("--kh-steps", type=int, default=0,
+                    help="kh: run this many steps instead of advancing to --t-end (0 = to --t-end)")
+    ap.add_argument("--t-compare", type=float, default=0.01,
+                    help="kh: early time of the mixed-vs-FP64 comparison (0 = none)")
     ap.add_argument("--mfma", type=int, default=-1, choices=[-1, 0, 1],
                     help="kh: the AV contraction on the matrix cores (1), on the VALU (0), or tuned (-1)")
     ap.add_argument("--instances", type=int, default=256)
@@ -288,8 +292,9 @@ def bench_kh(args):
     """The Kelvin-Helmholtz workload to --t-end (run_kh); the mixed form
     against the FP64 form at t = 0.01 and at the end time."""
     res = run_kh(args.K, args.p, args.forms.split(","), cfl=args.cfl, seed=args.seed, t_end=args.t_end,
+                 steps=args.kh_steps or None,
                  schedule=args.schedule, autotune=not args.no_autotune, mfma=args.mfma,
-                 shock=not args.no_shock_capturing)
+                 shock=not args.no_shock_capturing, t_compare=args.t_compare)
     print(json.dumps(res))
 
 
