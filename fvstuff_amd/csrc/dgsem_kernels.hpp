@@ -521,6 +521,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     // per SIMD (p = 7: 270 registers, 40 % slower), so they divide exactly
     constexpr bool NEWTON = sizeof(Real) == 8 && P <= 5;
     using Mx = M<Real, NEWTON>;
+    // the mixed kernels convert their nodes to primitives in FP32 (the FP64
+    // state rounded once) and, in the fused stages, take the element wave
+    // speeds the AV needs from those prims in FP32, the reference's precision
+    // for both (:396-409, :483-486); every wave speed that sets a dt stays
+    // FP64 (stage 3 below, and lam_el of RHS calls)
+    constexpr bool PRIM32 = sizeof(Real) == 4;
     extern __shared__ __align__(16) unsigned char smem[];
     // dt and the go flag go from thread 0 to the block through the first
     // bytes of the prims region, which the node phase writes only after every
@@ -757,14 +763,16 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         if (s >= E * N2) continue;
         const int e = s / N2, nd = s - e * N2;
         double W[4] = {1.0, 0.0, 0.0, 1.0}, ri = 1.0;
+        Prim<Real> w = {1, 0, 0, 1, 1};
         if (s < nE * N2) {
             if (UP_LDS && (staged || SD)) {
 #pragma unroll
                 for (int q = 0; q < 4; q++) sU[e * 4 * N2 + q * N2 + nd] = Ul[j][q];
             }
-            Cons64::to_prim(Ul[j], W, gs, ri);
+            if (PRIM32) w = ph.node_prim(Ul[j]);
+            else Cons64::to_prim(Ul[j], W, gs, ri);
         }
-        Prim<Real> w = make_prim<Real>(W);
+        if (!PRIM32) w = make_prim<Real>(W);
         Real* p = sW + s * 5;
         p[0] = (Real)0.5 * w.r; p[1] = (Real)0.5 * w.u; p[2] = (Real)0.5 * w.v;
         p[3] = (Real)0.5 * w.p; p[4] = (Real)0.5 * w.z;
@@ -776,8 +784,16 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             u[3 * N2] = w.p * ph.inv_gm1 + (Real)0.5 * w.r * (w.u * w.u + w.v * w.v);
         }
         if ((FULL || SD) && need_lam) {
-            const unsigned long long sp =
-                dbits(SD ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri));
+            double spd;
+            if (PRIM32 && !SD && staged) {
+                spd = (double)ph.node_speed(w);    // feeds the AV only
+            } else {
+                // lam_el of RHS calls (the dt of separate calls comes from it)
+                // stays FP64 in every form
+                if (PRIM32) Cons64::to_prim(Ul[j], W, gs, ri);
+                spd = SD ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri);
+            }
+            const unsigned long long sp = dbits(spd);
             if (!PAIR) {
                 elem_max_lds<N2>(&sLamE[e], sp, s < nE * N2);
             } else if (j & 1) {
@@ -908,8 +924,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                         double U[4], W[4];
 #pragma unroll
                         for (int q = 0; q < 4; q++) U[q] = f ? Hn[1][q] : Hn[0][q];
-                        Cons64::to_prim(U, W, gs);
-                        nb = make_prim<Real>(W);
+                        if (PRIM32) {
+                            nb = ph.node_prim(U);
+                        } else {
+                            Cons64::to_prim(U, W, gs);
+                            nb = make_prim<Real>(W);
+                        }
                         rot(nb, ydir);
                     }
                 }

@@ -204,6 +204,21 @@ template <typename Real, bool NEWTON = false> struct Phys {
         }
         return w;
     }
+    // cons_to_prim_sane of an FP64 node state in the compute precision, with
+    // z = rho/p: the state rounded once to Real (the precision the reference
+    // holds it in), then sanitized and converted in Real
+    __device__ __forceinline__ Prim<Real> node_prim(const double Ud[4]) const {
+        const Real U[4] = {(Real)Ud[0], (Real)Ud[1], (Real)Ud[2], (Real)Ud[3]};
+        Prim<Real> w = to_prim_sane(U);
+        w.z = w.r * Mx::rcp(w.p);
+        return w;
+    }
+    // node wave speed |u|+|v|+c of sanitized prims in the compute precision,
+    // with rho floored at 1e-12 (:483-486)
+    __device__ __forceinline__ Real node_speed(const Prim<Real>& w) const {
+        const Real c = Mx::sqrt_(Mx::fmax_(Mx::div(g * w.p, Mx::fmax_(w.r, (Real)1e-12)), (Real)1e-14));
+        return Mx::abs_(w.u) + Mx::abs_(w.v) + c;
+    }
     // physical flux along x (dir 0) or y (dir 1), :232-241
     template <int DIR>
     __device__ __forceinline__ void flux(const Prim<Real>& w, Real F[4]) const {
