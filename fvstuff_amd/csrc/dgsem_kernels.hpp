@@ -522,10 +522,10 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     constexpr bool NEWTON = sizeof(Real) == 8 && P <= 5;
     using Mx = M<Real, NEWTON>;
     // the mixed kernels convert their nodes to primitives in FP32 (the FP64
-    // state rounded once) and, in the fused stages, take the element wave
-    // speeds the AV needs from those prims in FP32, the reference's precision
-    // for both (:396-409, :483-486); every wave speed that sets a dt stays
-    // FP64 (stage 3 below, and lam_el of RHS calls)
+    // state rounded once) and take the node wave speeds of the fused stages
+    // (the AV's element maxima, stage 3's for the next dt) from those prims
+    // in FP32, the reference's precision for both (:396-409, :483-486); the
+    // maxima are FP64, and lam_el of RHS calls stays FP64 in every form
     constexpr bool PRIM32 = sizeof(Real) == 4;
     extern __shared__ __align__(16) unsigned char smem[];
     // dt and the go flag go from thread 0 to the block through the first
@@ -1077,7 +1077,11 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         return !(Us[0] >= a.eps_rho) ||
                !(Cons64::pressure_from_cons(Us, gs) >= 2.0 * a.eps_p + 1e-13 * fabs(Us[3]));
     };
+    // wave speed of a new node for the next dt (its maximum is taken in FP64);
+    // the mixed kernels compute it from the node's FP32 prims, as they do for
+    // the flux
     auto speed = [&](const double (&Us)[4]) {
+        if (PRIM32) return (double)ph.template node_speed<SD>(ph.node_prim(Us));
         double W[4], ri;
         Cons64::to_prim(Us, W, gs, ri);
         return SD ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri);

@@ -213,10 +213,13 @@ template <typename Real, bool NEWTON = false> struct Phys {
         w.z = w.r * Mx::rcp(w.p);
         return w;
     }
-    // node wave speed |u|+|v|+c of sanitized prims in the compute precision,
-    // with rho floored at 1e-12 (:483-486)
+    // node wave speed |u|+|v|+c of sanitized prims in the compute precision:
+    // the DGSEM form floors rho at 1e-12 (:483-486), the spectral-difference
+    // form takes sound_speed(p, rho) as it is (fr_lobatto_2d.cu:515-518)
+    template <bool SD = false>
     __device__ __forceinline__ Real node_speed(const Prim<Real>& w) const {
-        const Real c = Mx::sqrt_(Mx::fmax_(Mx::div(g * w.p, Mx::fmax_(w.r, (Real)1e-12)), (Real)1e-14));
+        const Real c = SD ? sound_speed(w.p, w.r)
+                          : Mx::sqrt_(Mx::fmax_(Mx::div(g * w.p, Mx::fmax_(w.r, (Real)1e-12)), (Real)1e-14));
         return Mx::abs_(w.u) + Mx::abs_(w.v) + c;
     }
     // physical flux along x (dir 0) or y (dir 1), :232-241
