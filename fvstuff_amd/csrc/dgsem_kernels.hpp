@@ -248,6 +248,35 @@ __device__ __forceinline__ void elem_max_lds(unsigned long long* dst, unsigned l
     }
 }
 
+// The same for a non-negative FP32 value (or a NaN, of either sign): its bit
+// pattern orders like the value as a 32-bit unsigned, so the DPP steps move
+// one register instead of two; the LDS atomic takes the FP64 bits of the
+// element maximum (FP32 -> FP64 is exact and keeps the order).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned umax32_dpp_step(unsigned v) {
+    const unsigned o = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROW_MASK, 0xf, false);
+    return o > v ? o : v;
+}
+__device__ __forceinline__ float fmax_bits(float a, float b) {
+    return __uint_as_float(max(__float_as_uint(a), __float_as_uint(b)));
+}
+template <int G>
+__device__ __forceinline__ void elem_max_lds(unsigned long long* dst, float f, bool live) {
+    const int lane = threadIdx.x & 63;
+    unsigned v = __float_as_uint(f);
+    if (G == 64 || G == 4 || G == 8 || G == 16 || G == 32) {
+        v = umax32_dpp_step<0x111, 0xf>(v);                    // row_shr:1
+        v = umax32_dpp_step<0x112, 0xf>(v);                    // row_shr:2
+        if (G >= 8) v = umax32_dpp_step<0x114, 0xf>(v);        // row_shr:4
+        if (G >= 16) v = umax32_dpp_step<0x118, 0xf>(v);       // row_shr:8
+        if (G >= 32) v = umax32_dpp_step<0x142, 0xa>(v);       // row_bcast:15 into rows 1 and 3
+        if (G == 64) v = umax32_dpp_step<0x143, 0xc>(v);       // row_bcast:31 into rows 2 and 3
+        if (live && (lane & (G - 1)) == G - 1) atomicMax(dst, dbits((double)__uint_as_float(v)));
+    } else if (live) {
+        atomicMax(dst, dbits((double)f));
+    }
+}
+
 // Block-wide max of v (FP64) -> one atomicMax on the bit pattern.
 __device__ __forceinline__ void block_max_atomic(double v, unsigned long long* dst, double* sred) {
     v = wave_max(v);
@@ -757,6 +786,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         __syncthreads();
     }
     unsigned long long lam_pair = 0ull;   // PAIR: wave speed of the pair's first node
+    float lam_pair32 = 0.f;
 #pragma unroll
     for (int j = 0; j < JMAX; j++) {
         const int s = slot(j);
@@ -783,17 +813,23 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             u[0] = w.r; u[N2] = w.r * w.u; u[2 * N2] = w.r * w.v;
             u[3 * N2] = w.p * ph.inv_gm1 + (Real)0.5 * w.r * (w.u * w.u + w.v * w.v);
         }
-        if ((FULL || SD) && need_lam) {
-            double spd;
-            if (PRIM32 && !SD && staged) {
-                spd = (double)ph.node_speed(w);    // feeds the AV only
+        if (PRIM32 && !SD && need_lam && staged) {
+            // FP32 node speeds, reduced on their 32-bit patterns (they feed
+            // the AV only)
+            const float sp = (float)ph.node_speed(w);
+            if (!PAIR) {
+                elem_max_lds<N2>(&sLamE[e], sp, s < nE * N2);
+            } else if (j & 1) {
+                elem_max_lds<N2 / 2>(&sLamE[e], fmax_bits(sp, lam_pair32), s < nE * N2);
             } else {
-                // lam_el of RHS calls (the dt of separate calls comes from it)
-                // stays FP64 in every form
-                if (PRIM32) Cons64::to_prim(Ul[j], W, gs, ri);
-                spd = SD ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri);
+                lam_pair32 = sp;
             }
-            const unsigned long long sp = dbits(spd);
+        } else if ((FULL || SD) && need_lam) {
+            // lam_el of RHS calls (the dt of separate calls comes from it)
+            // stays FP64 in every form
+            if (PRIM32) Cons64::to_prim(Ul[j], W, gs, ri);
+            const unsigned long long sp =
+                dbits(SD ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri));
             if (!PAIR) {
                 elem_max_lds<N2>(&sLamE[e], sp, s < nE * N2);
             } else if (j & 1) {
@@ -840,7 +876,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             Real rho = Mx::fmax_((Real)2 * p[0], (Real)1e-12), c2 = Mx::div(ph.g * ((Real)2 * p[3]), rho);
             Real divu = Mx::fmin_(jac * (sDu[s] + sDv[s]), (Real)0), du2 = divu * divu;
             Real th = Mx::div(du2, du2 + (Real)a.av_k * c2 * dx2inv + (Real)1e-30);
-            elem_max_lds<N2>(&sThE[e < E ? e : 0], dbits((double)th), live);
+            if (sizeof(Real) == 4) elem_max_lds<N2>(&sThE[e < E ? e : 0], (float)th, live);
+            else elem_max_lds<N2>(&sThE[e < E ? e : 0], dbits((double)th), live);
         }
         __syncthreads();
         if (tid < E)
