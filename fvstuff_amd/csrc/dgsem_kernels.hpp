@@ -1123,6 +1123,13 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         Cons64::to_prim(Us, W, gs, ri);
         return SD ? Cons64::wave_speed_sd(W, g, ri) : Cons64::wave_speed(W, g, ri);
     };
+    // With the limiter on, the mixed kernels keep the stage values in
+    // registers, and LDS takes them only on a tile that the limiter may
+    // change (see below); the FP64 kernels, with fewer registers to spare
+    // (p = 3: 170 instead of 153, a wave per SIMD less), write them to LDS
+    // at once.
+    constexpr bool LIM_REG = sizeof(Real) == 4;
+    double Sv[JMAX][4];
     if (PAIR) {
 #pragma unroll
         for (int j = 0; j < JMAX; j += 2) {
@@ -1135,8 +1142,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                 const int e = s / N2, nd = s - e * N2;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    sU[e * 4 * N2 + q * N2 + nd] = A[q];
-                    sU[e * 4 * N2 + q * N2 + nd + 1] = B[q];
+                    if (LIM_REG) {
+                        Sv[j][q] = A[q]; Sv[j + 1][q] = B[q];
+                    } else {
+                        sU[e * 4 * N2 + q * N2 + nd] = A[q];
+                        sU[e * 4 * N2 + q * N2 + nd + 1] = B[q];
+                    }
                 }
                 lim_bad = lim_bad || needs_limit(A) || needs_limit(B);
             } else {
@@ -1155,7 +1166,10 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             if (lim) {
                 const int e = s / N2, nd = s - e * N2;
 #pragma unroll
-                for (int q = 0; q < 4; q++) sU[e * 4 * N2 + q * N2 + nd] = Us[q];
+                for (int q = 0; q < 4; q++) {
+                    if (LIM_REG) Sv[j][q] = Us[q];
+                    else sU[e * 4 * N2 + q * N2 + nd] = Us[q];
+                }
                 lim_bad = lim_bad || needs_limit(Us);
             } else {
 #pragma unroll
@@ -1168,33 +1182,52 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         // With every node at rho >= eps_rho and p >= 2 eps_p (+ a relative
         // margin), the limiter's theta is 1 in every element of the tile and
         // it returns Ubar + (U - Ubar), U up to rounding: such a tile (on
-        // smooth and on most shocked data, every tile) skips it.  The
-        // workgroup reduction is also the barrier before the stage values
-        // are read back from LDS.
-        if (__syncthreads_or(lim_bad))
-            limit_block<N>(sU, a.w64, nE, g, a.eps_rho, a.eps_p, sBar, sTrho, sTheta);
-        if (PAIR) {
-            for (int s = 2 * tid; s < nE * N2; s += 2 * nthr) {
-                const int e = s / N2, nd = s - e * N2;
-                double A[4], B[4];
+        // smooth and on most shocked data, every tile) skips it, and its
+        // stage values go from registers to memory.  Otherwise they pass
+        // through LDS, where the limiter acts on whole elements.  The
+        // workgroup reduction is also the barrier after every read of the
+        // stage input in LDS, which the stage values overwrite.
+        const bool bad = __syncthreads_or(lim_bad);
+        if (bad) {
+            if (LIM_REG) {
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    A[q] = sU[e * 4 * N2 + q * N2 + nd];
-                    B[q] = sU[e * 4 * N2 + q * N2 + nd + 1];
-                    st2_at(outq[q], (unsigned)s * 8u, dbl2{A[q], B[q]});
+                for (int j = 0; j < JMAX; j++) {
+                    const int s = slot(j);
+                    if (s >= nE * N2) continue;
+                    const int e = s / N2, nd = s - e * N2;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) sU[e * 4 * N2 + q * N2 + nd] = Sv[j][q];
                 }
-                if (want_lambda) lmax = nanmax(nanmax(lmax, speed(A)), speed(B));
+            }
+            limit_block<N>(sU, a.w64, nE, g, a.eps_rho, a.eps_p, sBar, sTrho, sTheta);   // barriers on entry and exit
+        }
+        if (bad || !LIM_REG) {
+#pragma unroll
+            for (int j = 0; j < JMAX; j++) {
+                const int s = slot(j);
+                if (s >= nE * N2) continue;
+                const int e = s / N2, nd = s - e * N2;
+#pragma unroll
+                for (int q = 0; q < 4; q++) Sv[j][q] = sU[e * 4 * N2 + q * N2 + nd];
+            }
+        }
+        if (PAIR) {
+#pragma unroll
+            for (int j = 0; j < JMAX; j += 2) {
+                const int s = slot(j);
+                if (s >= nE * N2) continue;
+#pragma unroll
+                for (int q = 0; q < 4; q++) st2_at(outq[q], (unsigned)s * 8u, dbl2{Sv[j][q], Sv[j + 1][q]});
+                if (want_lambda) lmax = nanmax(nanmax(lmax, speed(Sv[j])), speed(Sv[j + 1]));
             }
         } else {
-            for (int s = tid; s < nE * N2; s += nthr) {
-                const int e = s / N2, nd = s - e * N2;
-                double Us[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    Us[q] = sU[e * 4 * N2 + q * N2 + nd];
-                    st_at(outq[q], (unsigned)s * 8u, Us[q]);
-                }
-                if (want_lambda) lmax = nanmax(lmax, speed(Us));
+            for (int j = 0; j < JMAX; j++) {
+                const int s = slot(j);
+                if (s >= nE * N2) continue;
+#pragma unroll
+                for (int q = 0; q < 4; q++) st_at(outq[q], (unsigned)s * 8u, Sv[j][q]);
+                if (want_lambda) lmax = nanmax(lmax, speed(Sv[j]));
             }
         }
     }
