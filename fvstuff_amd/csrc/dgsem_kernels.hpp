@@ -787,6 +787,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     }
     unsigned long long lam_pair = 0ull;   // PAIR: wave speed of the pair's first node
     float lam_pair32 = 0.f;
+    Real c2s[JMAX];                       // AV: c^2 of the thread's nodes, for the sensor
 #pragma unroll
     for (int j = 0; j < JMAX; j++) {
         const int s = slot(j);
@@ -803,6 +804,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             else Cons64::to_prim(Ul[j], W, gs, ri);
         }
         if (!PRIM32) w = make_prim<Real>(W);
+        if (FULL && !SD) c2s[j] = Mx::div(ph.g * w.p, Mx::fmax_(w.r, (Real)1e-12));
         Real* p = sW + s * 5;
         p[0] = (Real)0.5 * w.r; p[1] = (Real)0.5 * w.u; p[2] = (Real)0.5 * w.v;
         p[3] = (Real)0.5 * w.p; p[4] = (Real)0.5 * w.z;
@@ -863,21 +865,38 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         // Ducros-type sensor per node, max per element (:488-491)
         const Real len_p = kconst<Real>(a.len_p, a.len_p_f);
         const Real dx2inv = kconst<Real>(a.dx2inv, a.dx2inv_f);
+        // the node phase's slots (so c^2 comes from its registers) and its
+        // element reduction: every lane of an element's group takes part;
+        // slots past the tile write nothing
+        float th_pair32 = 0.f;
+        unsigned long long th_pair = 0ull;
 #pragma unroll
         for (int j = 0; j < JMAX; j++) {
-            // every lane takes part (the element reduction is wave-wide);
-            // slots past the tile read slot 0 and write nothing
-            const int s0 = tid + j * nthr;
-            const bool live = s0 < nE * N2;
-            const int s = live ? s0 : 0;
-            const int e = s0 / N2;
-            const Real* p = sW + s * 5;
+            const int s = slot(j);
+            if (s >= E * N2) continue;
+            const bool live = s < nE * N2;
+            const int e = s / N2;
             // (FP32: quotients through the hardware reciprocal, as in the fluxes)
-            Real rho = Mx::fmax_((Real)2 * p[0], (Real)1e-12), c2 = Mx::div(ph.g * ((Real)2 * p[3]), rho);
             Real divu = Mx::fmin_(jac * (sDu[s] + sDv[s]), (Real)0), du2 = divu * divu;
-            Real th = Mx::div(du2, du2 + (Real)a.av_k * c2 * dx2inv + (Real)1e-30);
-            if (sizeof(Real) == 4) elem_max_lds<N2>(&sThE[e < E ? e : 0], (float)th, live);
-            else elem_max_lds<N2>(&sThE[e < E ? e : 0], dbits((double)th), live);
+            Real th = Mx::div(du2, du2 + (Real)a.av_k * c2s[j] * dx2inv + (Real)1e-30);
+            if (sizeof(Real) == 4) {
+                if (!PAIR) {
+                    elem_max_lds<N2>(&sThE[e], (float)th, live);
+                } else if (j & 1) {
+                    elem_max_lds<N2 / 2>(&sThE[e], fmax_bits((float)th, th_pair32), live);
+                } else {
+                    th_pair32 = (float)th;
+                }
+            } else {
+                const unsigned long long tb = dbits((double)th);
+                if (!PAIR) {
+                    elem_max_lds<N2>(&sThE[e], tb, live);
+                } else if (j & 1) {
+                    elem_max_lds<N2 / 2>(&sThE[e], tb > th_pair ? tb : th_pair, live);
+                } else {
+                    th_pair = tb;
+                }
+            }
         }
         __syncthreads();
         if (tid < E)
