@@ -414,7 +414,7 @@ __device__ __forceinline__ void line_pairs(const PH& ph, const Prim<Real> (&hw)[
         for (int k = i + 1; k < N; k++) {
             Real F[4];
             if (VOL == VOL_RANOCHA) PH::ranocha_half(hw[i], hw[k], c, ph.inv_gm1, F);
-            else ph.template chandrashekar<(N <= 6)>(doubled(hw[i]), doubled(hw[k]), F);
+            else ph.template chandrashekar_half<(N <= 6)>(hw[i], hw[k], F);
             const Real dik = Dv[i * N + k], dki = Dv[k * N + i];
 #pragma unroll
             for (int q = 0; q < 4; q++) {

@@ -321,20 +321,24 @@ template <typename Real, bool NEWTON = false> struct Phys {
         F[3] = (Real)2 * (f1 * ka + kb) + f1 * izg;
     }
 
-    // Chandrashekar EC flux (:259-285) along x, beta = rho/(2p) = z/2.
+    // Chandrashekar EC flux (:259-285) along x, on half-scaled node states
+    // h = (rho, u, v, p, rho/p)/2 like ranocha_half: beta = rho/(2p) = h.z,
+    // the arithmetic means are sums of halves, and pair_means on the halved
+    // densities gives rho_ln/2 (both of its forms are homogeneous of degree
+    // one).  Scaling by 2 is exact, so this is the flux of the full states.
     // UNIFORM: wave-uniform log branch (lines of up to 6 nodes; at 7 and 8
     // nodes the branch form needs more registers than the kernels have)
     template <bool UNIFORM = true>
-    __device__ __forceinline__ void chandrashekar(const Prim<Real>& L, const Prim<Real>& R, Real F[4]) const {
-        Real bL = (Real)0.5 * L.z, bR = (Real)0.5 * R.z;
-        Real r_ln, inv_b_ln;
-        pair_means<UNIFORM>(L.r, R.r, bL, bR, r_ln, inv_b_ln);
+    __device__ __forceinline__ void chandrashekar_half(const Prim<Real>& L, const Prim<Real>& R, Real F[4]) const {
+        Real rh_ln, inv_b_ln;
+        pair_means<UNIFORM>(L.r, R.r, L.z, R.z, rh_ln, inv_b_ln);
         // p_hat = rho_avg/(2 beta_avg) = (rL+rR)/(2 (bL+bR))
-        Real p_hat = Mx::div((Real)0.5 * (L.r + R.r), bL + bR);
-        Real u_av = (Real)0.5 * (L.u + R.u), v_av = (Real)0.5 * (L.v + R.v);
-        Real vel2 = (Real)0.5 * (L.u * L.u + L.v * L.v) + (Real)0.5 * (R.u * R.u + R.v * R.v);
-        Real e_int = (Real)0.5 * (inv_gm1 * inv_b_ln - vel2);
-        Real f1 = r_ln * u_av, f2 = f1 * u_av + p_hat, f3 = f1 * v_av;
+        const Real p_hat = Mx::div(L.r + R.r, L.z + R.z);
+        const Real u_av = L.u + R.u, v_av = L.v + R.v;
+        // vel2 = (uL^2+vL^2)/2 + (uR^2+vR^2)/2 = 2 (sum of the halves' squares)
+        const Real hv2 = (L.u * L.u + L.v * L.v) + (R.u * R.u + R.v * R.v);
+        const Real e_int = (Real)0.5 * inv_gm1 * inv_b_ln - hv2;
+        const Real f1 = (Real)2 * rh_ln * u_av, f2 = f1 * u_av + p_hat, f3 = f1 * v_av;
         F[0] = f1; F[1] = f2; F[2] = f3; F[3] = f1 * e_int + f2 * u_av + f3 * v_av;
     }
 
