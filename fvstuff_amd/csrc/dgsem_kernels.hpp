@@ -214,11 +214,28 @@ __device__ __forceinline__ unsigned long long wave_umax_to_last(unsigned long lo
     v = umax_dpp_step<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
     return v;
 }
-// the maximum over the 64 lanes, in every lane
+// the same on 32-bit values (one register per step)
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned umax32_dpp_step(unsigned v) {
+    const unsigned o = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROW_MASK, 0xf, false);
+    return o > v ? o : v;
+}
+// 32-bit unsigned maximum over the 64 lanes, in every lane
+__device__ __forceinline__ unsigned wave_umax32(unsigned v) {
+    v = umax32_dpp_step<0x111, 0xf>(v);   // row_shr:1
+    v = umax32_dpp_step<0x112, 0xf>(v);   // row_shr:2
+    v = umax32_dpp_step<0x114, 0xf>(v);   // row_shr:4
+    v = umax32_dpp_step<0x118, 0xf>(v);   // row_shr:8
+    v = umax32_dpp_step<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v = umax32_dpp_step<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+// the maximum over the 64 lanes of 64-bit unsigned values, in every lane, as
+// two 32-bit passes: the high words, then the low words of the lanes that
+// hold the highest high word
 __device__ __forceinline__ unsigned long long wave_umax(unsigned long long v) {
-    v = wave_umax_to_last(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+    const unsigned hi = wave_umax32((unsigned)(v >> 32));
+    const unsigned lo = wave_umax32((unsigned)(v >> 32) == hi ? (unsigned)v : 0u);
     return ((unsigned long long)hi << 32) | lo;
 }
 
@@ -252,11 +269,6 @@ __device__ __forceinline__ void elem_max_lds(unsigned long long* dst, unsigned l
 // pattern orders like the value as a 32-bit unsigned, so the DPP steps move
 // one register instead of two; the LDS atomic takes the FP64 bits of the
 // element maximum (FP32 -> FP64 is exact and keeps the order).
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ unsigned umax32_dpp_step(unsigned v) {
-    const unsigned o = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROW_MASK, 0xf, false);
-    return o > v ? o : v;
-}
 __device__ __forceinline__ float fmax_bits(float a, float b) {
     return __uint_as_float(max(__float_as_uint(a), __float_as_uint(b)));
 }
@@ -1110,7 +1122,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     for (int q = 0; q < 4; q++) outq[q] = sbase(a.out + tile_off + q * S);
     // stage value of slot j (node s of the tile): Shu-Osher combination,
     // sanitize_cons, the SD repair
-    auto stage_value = [&](int j, int s, double (&Us)[4]) {
+    // (ke: the kinetic energy sanitize_cons computed, for the limiter test)
+    auto stage_value = [&](int j, int s, double (&Us)[4], double& ke) {
         const int e = s / N2, nd = s - e * N2;
         const double* up = sU + e * 4 * N2 + nd;
         double Ui[4];
@@ -1123,15 +1136,16 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
 #pragma unroll
             for (int q = 0; q < 4; q++) Us[q] = Ui[q] + dt * (double)rhs(e, nd, q);
         }
-        Cons64::sanitize_cons(Us, gs);
+        ke = Cons64::sanitize_cons(Us, gs);
         if (SD && a.limiter) Cons64::repair(Us, gs, a.eps_rho, a.eps_p);
     };
     // a stage value the Zhang-Shu limiter may change: rho < eps_rho, or a
     // pressure within a rounding margin of eps_p (NaN counts as such)
     bool lim_bad = false;
-    auto needs_limit = [&](const double (&Us)[4]) {
-        return !(Us[0] >= a.eps_rho) ||
-               !(Cons64::pressure_from_cons(Us, gs) >= 2.0 * a.eps_p + 1e-13 * fabs(Us[3]));
+    // (the pressure of a sanitized value from the kinetic energy sanitize_cons
+    // computed: pressure_from_cons, bit for bit)
+    auto needs_limit = [&](const double (&Us)[4], double ke) {
+        return !(Us[0] >= a.eps_rho) || !(gs.gm1 * (Us[3] - ke) >= 2.0 * a.eps_p + 1e-13 * fabs(Us[3]));
     };
     // wave speed of a new node for the next dt (its maximum is taken in FP64);
     // the mixed kernels compute it from the node's FP32 prims, as they do for
@@ -1155,8 +1169,9 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             const int s = slot(j);
             if (s >= nE * N2) continue;
             double A[4], B[4];
-            stage_value(j, s, A);
-            stage_value(j + 1, s + 1, B);
+            double keA, keB;
+            stage_value(j, s, A, keA);
+            stage_value(j + 1, s + 1, B, keB);
             if (lim) {
                 const int e = s / N2, nd = s - e * N2;
 #pragma unroll
@@ -1168,7 +1183,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                         sU[e * 4 * N2 + q * N2 + nd + 1] = B[q];
                     }
                 }
-                lim_bad = lim_bad || needs_limit(A) || needs_limit(B);
+                lim_bad = lim_bad || needs_limit(A, keA) || needs_limit(B, keB);
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; q++) st2_at(outq[q], (unsigned)s * 8u, dbl2{A[q], B[q]});
@@ -1181,7 +1196,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             const int s = slot(j);
             if (s >= nE * N2) continue;
             double Us[4];
-            stage_value(j, s, Us);
+            double ke;
+            stage_value(j, s, Us, ke);
             if (lim) {
                 const int e = s / N2, nd = s - e * N2;
 #pragma unroll
@@ -1189,7 +1205,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
                     if (LIM_REG) Sv[j][q] = Us[q];
                     else sU[e * 4 * N2 + q * N2 + nd] = Us[q];
                 }
-                lim_bad = lim_bad || needs_limit(Us);
+                lim_bad = lim_bad || needs_limit(Us, ke);
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; q++) st_at(outq[q], (unsigned)s * 8u, Us[q]);
@@ -1253,7 +1269,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     if (want_lambda) {
         // one atomic per wave on the ordered bit pattern (the speeds are
         // non-negative; a NaN, of either sign, orders above every number)
-        const unsigned long long m = wave_umax_to_last(dbits(lmax));
+        const unsigned long long m = wave_umax(dbits(lmax));
         if ((tid & 63) == 63) atomicMax(&a.st->lam_part[sc.slot_next][blockIdx.x & (LAM_PARTS - 1)], m);
     }
 }

@@ -101,10 +101,13 @@ struct Cons64 {
         double ke = 0.5 * (U[1] * U[1] + U[2] * U[2]) * rcp(rho);
         return gs.gm1 * (U[3] - ke);
     }
-    static __device__ __forceinline__ void sanitize_cons(double U[4], const Gas64& gs) {
+    // returns the kinetic energy it computed: pressure_from_cons of the
+    // result is gm1 * (U[3] - ke), bit for bit (the same rho, the same ke)
+    static __device__ __forceinline__ double sanitize_cons(double U[4], const Gas64& gs) {
         U[0] = fmax(U[0], 1e-14);
         double ke = 0.5 * (U[1] * U[1] + U[2] * U[2]) * rcp(U[0]);
         if (gs.gm1 * (U[3] - ke) < 1e-14) U[3] = gs.efloor + ke;
+        return ke;
     }
     // cons_to_prim_sane: sanitize, convert, floor rho/p, clamp |u|,|v| <= 1e4
     // (one reciprocal of rho serves u, v and the kinetic energy); also
