@@ -5,8 +5,10 @@
 //
 //  A  node phase  - each thread loads whole nodes (4 coalesced FP64 streams,
 //                   the reference's var-major SoA, dgsem_lobatto_2d.cu:126),
-//                   applies cons_to_prim_sane in FP64 and stores
-//                   (rho, u, v, p, rho/p) in the compute precision.
+//                   applies cons_to_prim_sane (FP64 kernels: in FP64; mixed
+//                   kernels: in FP32 on the node rounded once, as the
+//                   reference holds it) and stores (rho, u, v, p, rho/p) in
+//                   the compute precision.
 //  B  line phase  - one thread per grid line of an element (p+1 x-lines and
 //                   p+1 y-lines).  The thread keeps the line's p+1 states in
 //                   registers and evaluates every two-point volume flux of the
@@ -656,7 +658,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         for (int q = 0; q < 4; q++) Hn[f][q] = a.Up[q * S + o];
     }
 
-    // ── A: load nodes, prims in FP64, store compute-precision prims ──
+    // ── A: load nodes, prims (FP64 or FP32, see PRIM32), store compute-precision prims ──
     // A thread owns at most ceil((p+1)/2) node slots (E(p+1)^2 nodes over at
     // least 2E(p+1) threads), slot(j) for j < JMAX, here and in the update
     // phase.  All of its global loads are issued before any conversion, so
