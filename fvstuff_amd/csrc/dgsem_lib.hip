@@ -1208,6 +1208,9 @@ static hipError_t batch_step(dg_batch* b, StepCtl sc) {
     return hipSuccess;
 }
 
+// whether the batched kernel of a precision keeps the stage input in LDS
+static bool batch_up_lds(bool fp32) { return fp32 ? BatchOcc<float>::up_lds : BatchOcc<double>::up_lds; }
+
 // Fill the three per-stage argument blocks of every instance for states Qs.
 static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double t_end) {
     for (int i = 0; i < b->n; i++) {
@@ -1215,7 +1218,7 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
         const dg_config& f = c->cfg;
         const bool fp32 = f.precision == DG_MIXED;
         const int E = batch_elems_per_block(c->N, b->lanes);
-        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, !fp32, false, false, false);   // FP64: input in LDS
+        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(fp32), false, false, false);
         for (int s = 1; s <= 3; s++) {
             KArgs a;
             memset(&a, 0, sizeof(a));
@@ -1265,7 +1268,7 @@ static hipError_t batch_tiles(dg_batch* b) {
         ntl[i] = (c->NEL + E - 1) / E;
         tile_cost[i] = (double)E * c->N * c->N * c->N;
         const bool fp32 = b->common.precision == DG_MIXED;
-        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, !fp32, false, false, false);
+        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(fp32), false, false, false);
         lds = std::max(lds, (size_t)L.total);
     }
     // instances by tile cost, heaviest first (stable: ties keep their order)
