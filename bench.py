@@ -88,6 +88,9 @@ def parse():
     ap.add_argument("--persistent", type=int, default=-1, choices=[-1, 0, 1],
                     help="batched: persistent stage launches (1), one workgroup per tile (0), tuned (-1)")
     ap.add_argument("--lanes", type=int, default=0, help="batched: lanes per workgroup (0 = tuned)")
+    ap.add_argument("--by-degree", type=int, default=-1, choices=[-1, 0, 1],
+                    help="batched: one launch per degree with the single-solve kernels (1), the all-degree "
+                         "kernel (0), tuned (-1)")
     ap.add_argument("--xcd-bins", type=int, default=-1, choices=[-1, 0, 1],
                     help="batched: tile list in per-XCD bins (1), heaviest first (0), tuned (-1)")
     ap.add_argument("--seed", type=int, default=2024)
@@ -357,25 +360,29 @@ def bench_sd(args):
                       "unit": "DOF*stage/s", "dof": dof, **out}))
 
 
-def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, xcd_bins=-1, autotune=True):
+def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, xcd_bins=-1, autotune=True,
+                by_degree=-1):
     """Seeded ensemble of isentropic-vortex instances (K ~ U{16..128},
     p ~ U{3,5,7}, strength ~ U[1,5], Mach ~ U[0.1,0.8]) stepped in one
-    launch per stage for all instances vs instance by instance (wall clock
-    around each, after a synchronisation)."""
+    launch per stage for all instances (or one per degree present) vs
+    instance by instance (wall clock around each, after a synchronisation)."""
     import torch
     from fvstuff_amd import dgsem as D
     from fvstuff_amd.capi import ensemble
     cfgs, ics, cfls = ensemble(n=instances, seed=seed, precision=form)
     b = D.Batch(cfgs)
     Qs = b.init_states(ics)
-    tuned = autotune and not (lanes or persistent >= 0 or xcd_bins >= 0)
+    tuned = autotune and not (lanes or persistent >= 0 or xcd_bins >= 0 or by_degree >= 0)
     if tuned:
         b.autotune(Qs, cfls)
     else:
         b.set_options(use_graph=True, lanes=lanes)
+        b.set_by_degree(max(0, by_degree))
         b.set_xcd_bins(max(0, xcd_bins))
-        b.set_persistent(max(0, persistent))
+        if persistent > 0:
+            b.set_persistent(True)
     _, lanes, persistent, xcd_bins = b.options()
+    by_degree = b.by_degree()
     res = {}
     for mode in ("batched", "each"):
         Qs = b.init_states(ics, Qs)
@@ -391,7 +398,7 @@ def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, xc
         log(f"[bench-batched] {form} {mode}: {res[mode]}")
     return {"workload": f"{instances} vortex instances (seed {seed}), {form}", "unit": "DOF*stage/s",
             "dof": b.dofs, "steps": steps, "lanes_per_workgroup": lanes, "persistent": persistent,
-            "xcd_bins": xcd_bins, "tuned": tuned,
+            "xcd_bins": xcd_bins, "by_degree": by_degree, "tuned": tuned,
             "batched": res["batched"], "one_by_one": res["each"],
             "speedup": res["batched"]["value"] / res["each"]["value"]}
 
@@ -399,7 +406,7 @@ def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, xc
 def bench_batched(args):
     print(json.dumps(run_batched(args.instances, args.seed, args.forms.split(",")[-1], args.steps, args.warmup,
                                  lanes=args.lanes, persistent=args.persistent, xcd_bins=args.xcd_bins,
-                                 autotune=not args.no_autotune)))
+                                 autotune=not args.no_autotune, by_degree=args.by_degree)))
 
 
 def bench_copy(args):

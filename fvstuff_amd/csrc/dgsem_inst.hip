@@ -111,6 +111,72 @@ hipError_t DG_CAT(launch_stage_p, DG_P)(const KArgs& a, bool fp32, bool full, bo
     return full ? launch_vol<double, true>(a, grid, block, lds, s) : launch_vol<double, false>(a, grid, block, lds, s);
 }
 
+// Batched stage of one degree: tiles[] lists (instance, tile) pairs of the
+// instances of degree DG_P, an instance's tiles contiguous; one workgroup per
+// entry, in the XCD-aware order when xcd is set (each XCD takes a contiguous
+// range of the list, so an instance's tiles and the face nodes they read of
+// each other share an L2).  The body is the single-solve lean kernel's, with
+// its schedule and register budget for this degree alone (the all-degree
+// batched kernel carries the largest degree's), so every instance gets the
+// bits its own dg_advance would give.  The list and the argument blocks are
+// read through the constant address space: the kernel stores to global
+// memory, so the compiler cannot prove that plain global loads of them see
+// no store of the launch, and it loads the fields with vector loads into
+// VGPRs (p = 7 FP32: 168 VGPRs and 132 bytes of spills per lane); constant
+// loads are scalar and the body gets the single-solve kernel's registers
+// (164 VGPRs, no spills).  Both arrays are written only by the host.
+// (The host pass compiles the kernel as a stub and has no address spaces.)
+#ifdef __HIP_DEVICE_COMPILE__
+typedef __attribute__((address_space(4))) const KArgs ConstKArgs;
+typedef __attribute__((address_space(4))) const int2 ConstInt2;
+#else
+typedef const KArgs ConstKArgs;
+typedef const int2 ConstInt2;
+#endif
+// (The degree is a template argument: every object file instantiates this
+// kernel, and instantiations that differ only in DG_P would share one
+// symbol, of which the link keeps one.)
+template <int P, typename Real, int VOL>
+__global__ __launch_bounds__(BLOCK, (Occ<P, Real, VOL, false>::waves)) void batch_degree_kernel(
+    const KArgs* __restrict__ args, const int2* __restrict__ tiles, int ntiles, int stage, StepCtl sc, int xcd) {
+    const int t = tile_of_block(blockIdx.x, gridDim.x, xcd);
+    if (t >= ntiles) return;
+    const int2 tl = ((ConstInt2*)tiles)[t];
+    const KArgs a = ((ConstKArgs*)args)[(size_t)tl.x * 3 + (stage - 1)];
+    stage_tile<P, Real, false, VOL, (sizeof(Real) == 8), true, true, true, false>(a, sc, tl.y, tl.y == 0,
+                                                                                    (int)threadIdx.x);
+}
+
+template <typename Real, int VOL>
+static hipError_t launch_degree_one(const KArgs* args, const int2* tiles, int ntiles, int stage, StepCtl sc,
+                                    int block, size_t lds, int xcd, hipStream_t s) {
+    static size_t done = 0;
+    hipError_t e = set_lds(batch_degree_kernel<DG_P, Real, VOL>, lds, done);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((batch_degree_kernel<DG_P, Real, VOL>), dim3(ntiles), dim3(block), lds, s, args, tiles,
+                       ntiles, stage, sc, xcd);
+    return hipGetLastError();
+}
+
+template <typename Real>
+static hipError_t launch_degree_vol(const KArgs* args, const int2* tiles, int ntiles, int stage, StepCtl sc,
+                                    int vol, int block, size_t lds, int xcd, hipStream_t s) {
+    switch (vol) {
+        case VOL_RANOCHA: return launch_degree_one<Real, VOL_RANOCHA>(args, tiles, ntiles, stage, sc, block, lds, xcd, s);
+        case VOL_CHANDRASHEKAR:
+            return launch_degree_one<Real, VOL_CHANDRASHEKAR>(args, tiles, ntiles, stage, sc, block, lds, xcd, s);
+        default: return launch_degree_one<Real, VOL_COLLOCATION>(args, tiles, ntiles, stage, sc, block, lds, xcd, s);
+    }
+}
+
+hipError_t DG_CAT(launch_batch_degree_p, DG_P)(const KArgs* args, const int2* tiles, int ntiles, int stage,
+                                               StepCtl sc, bool fp32, int vol, int block, size_t lds, int xcd,
+                                               hipStream_t s) {
+    if (ntiles < 1) return hipSuccess;
+    return fp32 ? launch_degree_vol<float>(args, tiles, ntiles, stage, sc, vol, block, lds, xcd, s)
+                : launch_degree_vol<double>(args, tiles, ntiles, stage, sc, vol, block, lds, xcd, s);
+}
+
 hipError_t DG_CAT(launch_limit_p, DG_P)(double* Q, const double* w64, int NEL, int E, double g, double eps_rho,
                                         double eps_p, hipStream_t s) {
     constexpr int N2 = (DG_P + 1) * (DG_P + 1);

@@ -524,6 +524,12 @@ static LimitLauncher limit_launcher(int p) {
                                       launch_limit_p4, launch_limit_p5, launch_limit_p6, launch_limit_p7};
     return t[p];
 }
+static BatchDegreeLauncher batch_degree_launcher(int p) {
+    static const BatchDegreeLauncher t[] = {nullptr, launch_batch_degree_p1, launch_batch_degree_p2,
+                                            launch_batch_degree_p3, launch_batch_degree_p4, launch_batch_degree_p5,
+                                            launch_batch_degree_p6, launch_batch_degree_p7};
+    return t[p];
+}
 static SensorLauncher sensor_launcher(int p) {
     static const SensorLauncher t[] = {nullptr, launch_sensor_p1, launch_sensor_p2, launch_sensor_p3,
                                        launch_sensor_p4, launch_sensor_p5, launch_sensor_p6, launch_sensor_p7};
@@ -1170,8 +1176,15 @@ struct dg_batch {
     size_t lds = 0;
     bool use_graph = true;
     bool persistent = false;            // persistent form of the stage launches
-    bool xcd_bins = false;              // tile list in per-XCD bins (batch_tiles)
+    bool xcd_bins = false;              // tile list in per-XCD bins (batch_tiles); by degree: XCD-aware order
     int pgrid = 0;                      // its grid: resident workgroups (0 = unavailable)
+    // by-degree form: one launch per degree present per stage (batch_tiles)
+    struct DegreeRun {
+        int p, first, count, block;     // degree, its range of the tile list, workgroup size
+        size_t lds;
+    };
+    bool by_degree = false;
+    std::vector<DegreeRun> runs;
     hipGraphExec_t gexec = nullptr;
     char err[512] = {0};
 };
@@ -1194,6 +1207,15 @@ static int bfail(dg_batch* b, int code, const char* fmt, ...) {
 
 static hipError_t batch_launch_stage(dg_batch* b, int stage, StepCtl sc) {
     const bool fp32 = b->common.precision == DG_MIXED;
+    if (b->by_degree) {
+        for (const dg_batch::DegreeRun& r : b->runs) {
+            hipError_t e = batch_degree_launcher(r.p)(b->d_args, b->d_tiles + r.first, r.count, stage, sc, fp32,
+                                                      b->common.volume_flux, r.block, r.lds, b->xcd_bins ? 1 : 0,
+                                                      b->cap);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     const int grid = b->persistent ? b->pgrid : 0;
     return fp32 ? launch_batch_stage_float(b->d_args, b->d_tiles, b->ntiles, stage, sc, b->common.volume_flux,
                                            b->lanes, b->lds, grid, b->cap)
@@ -1208,8 +1230,12 @@ static hipError_t batch_step(dg_batch* b, StepCtl sc) {
     return hipSuccess;
 }
 
-// whether the batched kernel of a precision keeps the stage input in LDS
-static bool batch_up_lds(bool fp32) { return fp32 ? BatchOcc<float>::up_lds : BatchOcc<double>::up_lds; }
+// whether the batched kernels keep the stage input in LDS: the all-degree
+// kernel by precision, the by-degree form always (the single-solve schedule)
+static bool batch_up_lds(const dg_batch* b) {
+    if (b->by_degree) return true;
+    return b->common.precision == DG_MIXED ? BatchOcc<float>::up_lds : BatchOcc<double>::up_lds;
+}
 
 // Fill the three per-stage argument blocks of every instance for states Qs.
 static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double t_end) {
@@ -1218,7 +1244,7 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
         const dg_config& f = c->cfg;
         const bool fp32 = f.precision == DG_MIXED;
         const int E = batch_elems_per_block(c->N, b->lanes);
-        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(fp32), false, false, false);
+        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(b), false, false, false);
         for (int s = 1; s <= 3; s++) {
             KArgs a;
             memset(&a, 0, sizeof(a));
@@ -1257,25 +1283,43 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
 //    instance -1).  The hardware deals workgroups round-robin over the XCDs,
 //    so XCD x runs bin x in order: the tiles of an instance, which read each
 //    other's face nodes, share an L2 and run close in time.
+// The by-degree form lists the instances of each degree together, highest
+// degree first, each instance's tiles contiguous and in order; each degree's
+// range is one launch (b->runs).
 static hipError_t batch_tiles(dg_batch* b) {
     std::vector<int2> tiles;
     size_t lds = 0;
     std::vector<double> tile_cost(b->n);
     std::vector<int> ntl(b->n);
+    const bool fp32 = b->common.precision == DG_MIXED;
     for (int i = 0; i < b->n; i++) {
         dg_ctx* c = b->ctx[i];
         const int E = batch_elems_per_block(c->N, b->lanes);
         ntl[i] = (c->NEL + E - 1) / E;
         tile_cost[i] = (double)E * c->N * c->N * c->N;
-        const bool fp32 = b->common.precision == DG_MIXED;
-        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(fp32), false, false, false);
+        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(b), false, false, false);
         lds = std::max(lds, (size_t)L.total);
     }
     // instances by tile cost, heaviest first (stable: ties keep their order)
     std::vector<int> order(b->n);
     for (int i = 0; i < b->n; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return tile_cost[x] > tile_cost[y]; });
-    if (!b->xcd_bins) {
+    b->runs.clear();
+    if (b->by_degree) {
+        for (int p = PMAX_GPU; p >= 1; p--) {
+            dg_batch::DegreeRun r{p, (int)tiles.size(), 0, 0, 0};
+            for (int i = 0; i < b->n; i++) {
+                dg_ctx* c = b->ctx[i];
+                if (c->cfg.p != p) continue;
+                for (int t = 0; t < ntl[i]; t++) tiles.push_back(make_int2(i, t));
+                const int E = batch_elems_per_block(c->N, b->lanes);
+                r.block = std::min(BLOCK, (E * 2 * c->N + 63) / 64 * 64);
+                r.lds = (size_t)lds_layout(c->N, E, fp32 ? 4 : 8, true, false, false, false).total;
+            }
+            r.count = (int)tiles.size() - r.first;
+            if (r.count > 0) b->runs.push_back(r);
+        }
+    } else if (!b->xcd_bins) {
         for (int i : order)
             for (int t = 0; t < ntl[i]; t++) tiles.push_back(make_int2(i, t));
     } else {
@@ -1438,6 +1482,11 @@ int dg_batch_set_xcd_bins(dg_batch* b, int xcd_bins);
 
 int dg_batch_set_persistent(dg_batch* b, int persistent) {
     if (!b) return DG_ERR_INVALID_ARG;
+    if (persistent && b->by_degree) {   // the persistent form is the all-degree kernel's
+        b->by_degree = false;
+        BHIPCK(b, batch_tiles(b));
+        if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    }
     if (persistent && b->pgrid < 1)
         return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_set_persistent: no resident workgroup at this size");
     if ((persistent != 0) != b->persistent && b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
@@ -1510,17 +1559,18 @@ int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* la
     hipEventCreate(&e1);
     float best = 1e30f;
     int best_lanes = b->lanes;
-    bool best_pers = b->persistent;
+    bool best_pers = b->persistent, best_deg = b->by_degree;
     int rc = DG_OK;
     bool best_bins = b->xcd_bins;
-    // each size with both list orders, each in both launch forms (one
-    // workgroup per tile, persistent)
-    for (int cand = 0; cand < 12; cand++) {
-        const int l = cands[cand >> 2];
+    // each size with both orders in the three launch forms (the all-degree
+    // kernel with one workgroup per tile or persistent, one launch per degree)
+    for (int cand = 0; cand < 18; cand++) {
+        const int l = cands[cand / 6], form = (cand % 6) >> 1, bins = cand & 1;
         if ((rc = dg_batch_set_options(b, b->use_graph, l))) break;
-        if ((rc = dg_batch_set_xcd_bins(b, (cand >> 1) & 1))) break;
-        if ((cand & 1) && b->pgrid < 1) continue;
-        if ((rc = dg_batch_set_persistent(b, cand & 1))) break;
+        if ((rc = dg_batch_set_by_degree(b, form == 2))) break;
+        if ((rc = dg_batch_set_xcd_bins(b, bins))) break;
+        if (form == 1 && b->pgrid < 1) continue;
+        if ((rc = dg_batch_set_persistent(b, form == 1))) break;
         // note: the states are advanced by 4 steps per candidate at cfl[i]
         if ((rc = dg_batch_advance(b, Qs, 1, cfl, 0.0, nullptr, nullptr))) break;   // warm-up + graph
         hipEventRecord(e0, b->stream);
@@ -1529,11 +1579,14 @@ int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* la
         hipEventSynchronize(e1);
         float ms = 0.f;
         hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best) { best = ms; best_lanes = l; best_pers = b->persistent; best_bins = b->xcd_bins; }
+        if (ms < best) {
+            best = ms; best_lanes = l; best_pers = b->persistent; best_bins = b->xcd_bins; best_deg = b->by_degree;
+        }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     int rc2 = dg_batch_set_options(b, b->use_graph, best_lanes);
+    if (!rc2) rc2 = dg_batch_set_by_degree(b, best_deg);
     if (!rc2) rc2 = dg_batch_set_xcd_bins(b, best_bins);
     if (!rc2) rc2 = dg_batch_set_persistent(b, best_pers);
     if (lanes) *lanes = b->lanes;
@@ -1546,6 +1599,22 @@ int dg_batch_get_options(const dg_batch* b, int* use_graph, int* lanes, int* per
     if (lanes) *lanes = b->lanes;
     if (persistent) *persistent = b->persistent;
     if (xcd_bins) *xcd_bins = b->xcd_bins;
+    return DG_OK;
+}
+
+int dg_batch_set_by_degree(dg_batch* b, int by_degree) {
+    if (!b) return DG_ERR_INVALID_ARG;
+    if ((by_degree != 0) == b->by_degree) return DG_OK;
+    b->by_degree = by_degree != 0;
+    if (b->by_degree) b->persistent = false;
+    BHIPCK(b, batch_tiles(b));
+    if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }   // the launches are baked into the graph
+    return DG_OK;
+}
+
+int dg_batch_get_by_degree(const dg_batch* b, int* by_degree) {
+    if (!b || !by_degree) return DG_ERR_INVALID_ARG;
+    *by_degree = b->by_degree;
     return DG_OK;
 }
 

@@ -65,6 +65,8 @@ SIGNATURES = {
     "dg_batch_autotune": (C.c_int, [_vp, C.POINTER(_vp), _dp, C.POINTER(C.c_int)]),
     "dg_batch_set_persistent": (C.c_int, [_vp, C.c_int]),
     "dg_batch_set_xcd_bins": (C.c_int, [_vp, C.c_int]),
+    "dg_batch_set_by_degree": (C.c_int, [_vp, C.c_int]),
+    "dg_batch_get_by_degree": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "dg_batch_get_options": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),
 }
@@ -280,7 +282,8 @@ class Solver:
 
 class Batch:
     """Many independent solves (own K, p, domain, IC, CFL) stepped together:
-    one launch per stage for all instances (one workgroup per tile), graph-replayed."""
+    one launch per stage for all instances (one workgroup per tile), or one per
+    degree present (set_by_degree), graph-replayed."""
 
     def __init__(self, cfgs, device="cuda:0", stream=None):
         if not torch.cuda.is_available():
@@ -356,6 +359,16 @@ class Batch:
     def set_xcd_bins(self, xcd_bins):
         """Tile list in per-XCD bins of instances (True) or all tiles heaviest first."""
         self._check(self.lib.dg_batch_set_xcd_bins(self._b, int(bool(xcd_bins))))
+
+    def set_by_degree(self, by_degree):
+        """One launch per degree present per stage, each with the single-solve
+        kernel of its degree (True), or the all-degree kernel (False)."""
+        self._check(self.lib.dg_batch_set_by_degree(self._b, int(bool(by_degree))))
+
+    def by_degree(self):
+        v = C.c_int()
+        self._check(self.lib.dg_batch_get_by_degree(self._b, C.byref(v)))
+        return bool(v.value)
 
     def options(self):
         """(use_graph, lanes, persistent, xcd_bins) currently in use."""

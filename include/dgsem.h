@@ -269,7 +269,9 @@ int         dg_autotune(dg_ctx* ctx, int* elems_per_block, int* xcd_remap);
  * one solve per process): ensembles of instances that differ in K, p, domain,
  * IC and CFL but share the flux / boundary / precision switches.  Each stage
  * of all instances is one launch over a tile list (heaviest tiles first, one
- * workgroup per tile, every degree's code in the same kernel); each instance
+ * workgroup per tile, every degree's code in the same kernel), or, in the
+ * by-degree form, one launch per degree present (the instances of that
+ * degree, each with the single-solve kernel of that degree); each instance
  * keeps its own device-side dt and time.  The stage launches of the time loop
  * are replayed from a captured HIP graph.  AV and the limiter are not
  * available in batched mode (DG_ERR_INVALID_ARG). */
@@ -302,11 +304,21 @@ int         dg_batch_set_persistent(dg_batch* b, int persistent);
  * the bins interleaved so that each XCD runs one bin and the tiles of an
  * instance share an L2).  Results are identical either way. */
 int         dg_batch_set_xcd_bins(dg_batch* b, int xcd_bins);
+/* By-degree form (1): each stage is one launch per degree present, over the
+ * tiles of the instances of that degree (an instance's tiles contiguous,
+ * highest degree first), each launch running the single-solve lean kernel of
+ * its degree with its own register budget and schedule; xcd_bins then selects
+ * the XCD-aware block order within each launch.  0 = the all-degree kernel
+ * (the default).  The by-degree and persistent forms exclude each other:
+ * setting one clears the other.  Results are identical in every form. */
+int         dg_batch_set_by_degree(dg_batch* b, int by_degree);
+int         dg_batch_get_by_degree(const dg_batch* b, int* by_degree);
 /* Current options (any pointer may be NULL). */
 int         dg_batch_get_options(const dg_batch* b, int* use_graph, int* lanes, int* persistent, int* xcd_bins);
-/* Time the three workgroup sizes, each with both list orders and both
- * launch forms, on the given states (they are advanced by a few steps) and
- * keep the fastest; writes the size to *lanes if non-NULL. */
+/* Time the three workgroup sizes, each with both orders in the three launch
+ * forms (all-degree kernel one workgroup per tile, persistent, by degree), on
+ * the given states (they are advanced by a few steps) and keep the fastest;
+ * writes the size to *lanes if non-NULL. */
 int         dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* lanes);
 
 #ifdef __cplusplus

@@ -34,6 +34,17 @@ def test_library_exports_every_symbol():
         assert re.search(rf"\bT {name}$", out, re.M), name
 
 
+def test_per_degree_kernels_distinct():
+    """Every kernel template compiled in the per-degree objects carries the
+    degree among its template arguments, so the link keeps one instantiation
+    per degree (a template without it is instantiated with the same symbol in
+    all seven objects, and only one of them survives the link)."""
+    out = subprocess.run(["nm", "-C", LIB_PATH], capture_output=True, text=True).stdout
+    for kern in ("stage_kernel", "batch_degree_kernel", "limit_kernel", "sensor_kernel"):
+        degrees = {int(m) for m in re.findall(rf"dg::{kern}<(\d+)\b", out)}
+        assert degrees == set(range(1, 8)), (kern, sorted(degrees))
+
+
 def test_host_only_calls():
     lib = load_library()
     cfg = capi.DGConfig()

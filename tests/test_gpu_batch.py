@@ -95,19 +95,23 @@ def test_batch_rejects_av_and_limiter(D):
         D.Batch([cfg])
 
 
+@pytest.mark.parametrize("by_degree", [False, True], ids=["all-degree", "by-degree"])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-6)])
-def test_batch_matches_oracle(D, precision, tol):
+def test_batch_matches_oracle(D, precision, tol, by_degree):
     """Every instance of a 12-instance ensemble (degrees 1..7, K 4..20, seeded
     strength and Mach) after 20 batched steps against the FP64 oracle run
     of the same instance: FP64 within 1e-12 relative L2, mixed within 1e-6.
-    The batched kernel has its own load schedule (in FP32 the stage input is
-    loaded twice instead of kept in LDS, and U^n after the line phase)."""
+    The all-degree kernel has its own load schedule (in FP32 the stage input
+    is loaded twice instead of kept in LDS, and U^n after the line phase);
+    the by-degree form runs one launch per degree with the single-solve body."""
     from oracle.oracle import Oracle, lgl
     from tests.layout import gll_weights_2d, rel_l2
     # seed 4 draws every degree from 1 to 7
     cfgs, ics, cfls = ensemble(n=12, seed=4, precision=precision, k_range=(4, 20), degrees=(1, 2, 3, 4, 5, 6, 7))
     assert sorted({c.p for c in cfgs}) == [1, 2, 3, 4, 5, 6, 7]
     b = D.Batch(cfgs)
+    b.set_by_degree(by_degree)
+    assert b.by_degree() == by_degree
     Qs = b.init_states(ics)
     ts = b.advance(Qs, 20, cfls)
     got = host(Qs)
@@ -169,3 +173,59 @@ def test_batch_persistent_matches(D, precision):
                     assert got[0] == ref[0], key
                     for a, c in zip(got[1], ref[1]):
                         assert np.array_equal(a, c), key
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_batch_by_degree_matches(D, precision):
+    """The by-degree form (one launch per degree present, each with the
+    single-solve lean kernel of its degree) gives every instance the bits of
+    its own single solve, and so of the all-degree kernel, at every workgroup
+    size, in both block orders, with and without graph replay; per-instance
+    end times are honoured."""
+    cfgs, ics, cfls = ensemble(n=11, seed=4, precision=precision, k_range=(4, 20), degrees=(1, 2, 3, 4, 5, 6, 7))
+    ref = []
+    for cfg, ic, cfl in zip(cfgs, ics, cfls):
+        s = D.Solver(cfg)
+        Q = s.init_state(ic)
+        t, _ = s.advance(Q, 7, cfl)
+        ref.append((t, host([Q])[0]))
+    b = D.Batch(cfgs)
+    for lanes in (64, 128, 256):
+        for bins in (False, True):
+            for use_graph in (True, False):
+                b.set_options(use_graph=use_graph, lanes=lanes)
+                b.set_by_degree(True)
+                b.set_xcd_bins(bins)
+                assert b.by_degree() and b.options() == (use_graph, lanes, False, bins)
+                Qs = b.init_states(ics)
+                ts = b.advance(Qs, 7, cfls)
+                got = host(Qs)
+                key = (lanes, bins, use_graph)
+                for i, (t, Q) in enumerate(ref):
+                    assert ts[i] == t, (key, i)
+                    assert np.array_equal(got[i], Q), (key, i)
+    # a common end time inside the run: every instance stops there
+    b.set_options(use_graph=True, lanes=128)
+    Qs = b.init_states(ics)
+    ts = b.advance(Qs, 100000, cfls, t_end=0.01)
+    assert all(t == pytest.approx(0.01, abs=1e-13) for t in ts)
+
+
+def test_batch_forms_exclude_each_other(D):
+    """The by-degree and persistent forms exclude each other: setting one
+    clears the other, and each gives the same bits."""
+    cfgs, ics, cfls = ensemble(n=6, seed=9, k_range=(4, 12), degrees=(2, 3, 5))
+    b = D.Batch(cfgs)
+    out = []
+    b.set_by_degree(True)
+    b.set_persistent(True)
+    assert not b.by_degree() and b.options()[2]
+    Qs = b.init_states(ics)
+    out.append((b.advance(Qs, 5, cfls), host(Qs)))
+    b.set_by_degree(True)
+    assert b.by_degree() and not b.options()[2]
+    Qs = b.init_states(ics)
+    out.append((b.advance(Qs, 5, cfls), host(Qs)))
+    assert out[0][0] == out[1][0]
+    for a, c in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, c)
