@@ -383,6 +383,7 @@ def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, xc
             b.set_persistent(True)
     _, lanes, persistent, xcd_bins = b.options()
     by_degree = b.by_degree()
+    lanes_by_degree = {p: b.degree_lanes(p) for p in sorted({c.p for c in cfgs})} if by_degree else None
     res = {}
     for mode in ("batched", "each"):
         Qs = b.init_states(ics, Qs)
@@ -398,7 +399,7 @@ def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, xc
         log(f"[bench-batched] {form} {mode}: {res[mode]}")
     return {"workload": f"{instances} vortex instances (seed {seed}), {form}", "unit": "DOF*stage/s",
             "dof": b.dofs, "steps": steps, "lanes_per_workgroup": lanes, "persistent": persistent,
-            "xcd_bins": xcd_bins, "by_degree": by_degree, "tuned": tuned,
+            "xcd_bins": xcd_bins, "by_degree": by_degree, "lanes_by_degree": lanes_by_degree, "tuned": tuned,
             "batched": res["batched"], "one_by_one": res["each"],
             "speedup": res["batched"]["value"] / res["each"]["value"]}
 

@@ -1185,6 +1185,7 @@ struct dg_batch {
     };
     bool by_degree = false;
     std::vector<DegreeRun> runs;
+    int deg_lanes[PMAX_GPU + 1] = {0};  // by-degree form: lanes per workgroup of degree p (0 = lanes)
     hipGraphExec_t gexec = nullptr;
     char err[512] = {0};
 };
@@ -1230,6 +1231,12 @@ static hipError_t batch_step(dg_batch* b, StepCtl sc) {
     return hipSuccess;
 }
 
+// lanes per workgroup of an instance of degree p: its own in the by-degree
+// form when set, else the common size (the all-degree kernel has one size)
+static int batch_lanes_of(const dg_batch* b, int p) {
+    return b->by_degree && b->deg_lanes[p] ? b->deg_lanes[p] : b->lanes;
+}
+
 // whether the batched kernels keep the stage input in LDS: the all-degree
 // kernel by precision, the by-degree form always (the single-solve schedule)
 static bool batch_up_lds(const dg_batch* b) {
@@ -1243,7 +1250,7 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
         dg_ctx* c = b->ctx[i];
         const dg_config& f = c->cfg;
         const bool fp32 = f.precision == DG_MIXED;
-        const int E = batch_elems_per_block(c->N, b->lanes);
+        const int E = batch_elems_per_block(c->N, batch_lanes_of(b, c->cfg.p));
         LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(b), false, false, false);
         for (int s = 1; s <= 3; s++) {
             KArgs a;
@@ -1294,7 +1301,7 @@ static hipError_t batch_tiles(dg_batch* b) {
     const bool fp32 = b->common.precision == DG_MIXED;
     for (int i = 0; i < b->n; i++) {
         dg_ctx* c = b->ctx[i];
-        const int E = batch_elems_per_block(c->N, b->lanes);
+        const int E = batch_elems_per_block(c->N, batch_lanes_of(b, c->cfg.p));
         ntl[i] = (c->NEL + E - 1) / E;
         tile_cost[i] = (double)E * c->N * c->N * c->N;
         LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(b), false, false, false);
@@ -1312,7 +1319,7 @@ static hipError_t batch_tiles(dg_batch* b) {
                 dg_ctx* c = b->ctx[i];
                 if (c->cfg.p != p) continue;
                 for (int t = 0; t < ntl[i]; t++) tiles.push_back(make_int2(i, t));
-                const int E = batch_elems_per_block(c->N, b->lanes);
+                const int E = batch_elems_per_block(c->N, batch_lanes_of(b, c->cfg.p));
                 r.block = std::min(BLOCK, (E * 2 * c->N + 63) / 64 * 64);
                 r.lds = (size_t)lds_layout(c->N, E, fp32 ? 4 : 8, true, false, false, false).total;
             }
@@ -1557,38 +1564,66 @@ int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* la
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
+    // one step (warm-up, graph instantiation), then 3 timed steps; the
+    // states are advanced by 4 steps per candidate at cfl[i]
+    auto time_steps = [&](float* ms) -> int {
+        int r = dg_batch_advance(b, Qs, 1, cfl, 0.0, nullptr, nullptr);
+        if (r) return r;
+        hipEventRecord(e0, b->stream);
+        if ((r = dg_batch_advance(b, Qs, 3, cfl, 0.0, nullptr, nullptr))) return r;
+        hipEventRecord(e1, b->stream);
+        hipEventSynchronize(e1);
+        *ms = 0.f;
+        hipEventElapsedTime(ms, e0, e1);
+        return DG_OK;
+    };
     float best = 1e30f;
     int best_lanes = b->lanes;
     bool best_pers = b->persistent, best_deg = b->by_degree;
     int rc = DG_OK;
     bool best_bins = b->xcd_bins;
+    for (int p = 1; p <= PMAX_GPU && !rc; p++) rc = dg_batch_set_degree_lanes(b, p, 0);
     // each size with both orders in the three launch forms (the all-degree
     // kernel with one workgroup per tile or persistent, one launch per degree)
-    for (int cand = 0; cand < 18; cand++) {
+    for (int cand = 0; cand < 18 && !rc; cand++) {
         const int l = cands[cand / 6], form = (cand % 6) >> 1, bins = cand & 1;
         if ((rc = dg_batch_set_options(b, b->use_graph, l))) break;
         if ((rc = dg_batch_set_by_degree(b, form == 2))) break;
         if ((rc = dg_batch_set_xcd_bins(b, bins))) break;
         if (form == 1 && b->pgrid < 1) continue;
         if ((rc = dg_batch_set_persistent(b, form == 1))) break;
-        // note: the states are advanced by 4 steps per candidate at cfl[i]
-        if ((rc = dg_batch_advance(b, Qs, 1, cfl, 0.0, nullptr, nullptr))) break;   // warm-up + graph
-        hipEventRecord(e0, b->stream);
-        if ((rc = dg_batch_advance(b, Qs, 3, cfl, 0.0, nullptr, nullptr))) break;
-        hipEventRecord(e1, b->stream);
-        hipEventSynchronize(e1);
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
+        float ms;
+        if ((rc = time_steps(&ms))) break;
         if (ms < best) {
             best = ms; best_lanes = l; best_pers = b->persistent; best_bins = b->xcd_bins; best_deg = b->by_degree;
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     int rc2 = dg_batch_set_options(b, b->use_graph, best_lanes);
     if (!rc2) rc2 = dg_batch_set_by_degree(b, best_deg);
     if (!rc2) rc2 = dg_batch_set_xcd_bins(b, best_bins);
     if (!rc2) rc2 = dg_batch_set_persistent(b, best_pers);
+    // by-degree form: each degree's launch is independent of the others, so
+    // its workgroup size is refined on its own, the other degrees fixed; a
+    // size is kept only if the whole step runs at least 1 % faster with it
+    if (!rc && !rc2 && best_deg) {
+        std::vector<int> degs;
+        for (const dg_batch::DegreeRun& r : b->runs) degs.push_back(r.p);
+        float ref;
+        rc = time_steps(&ref);   // the kept configuration on the states as they are now
+        for (size_t k = 0; k < degs.size() && !rc; k++) {
+            const int p = degs[k];
+            int keep = 0;
+            for (int l : cands) {
+                if (l == best_lanes) continue;
+                float ms;
+                if ((rc = dg_batch_set_degree_lanes(b, p, l)) || (rc = time_steps(&ms))) break;
+                if (ms < 0.99f * ref) { ref = ms; keep = l; }
+            }
+            if (!rc) rc = dg_batch_set_degree_lanes(b, p, keep);
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
     if (lanes) *lanes = b->lanes;
     return rc ? rc : rc2;
 }
@@ -1609,6 +1644,25 @@ int dg_batch_set_by_degree(dg_batch* b, int by_degree) {
     if (b->by_degree) b->persistent = false;
     BHIPCK(b, batch_tiles(b));
     if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }   // the launches are baked into the graph
+    return DG_OK;
+}
+
+int dg_batch_set_degree_lanes(dg_batch* b, int p, int lanes) {
+    if (!b) return DG_ERR_INVALID_ARG;
+    if (p < 1 || p > PMAX_GPU || (lanes != 0 && lanes != 64 && lanes != 128 && lanes != 256))
+        return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_set_degree_lanes: degree %d, lanes %d", p, lanes);
+    if (b->deg_lanes[p] == lanes) return DG_OK;
+    b->deg_lanes[p] = lanes;
+    if (b->by_degree) {
+        BHIPCK(b, batch_tiles(b));
+        if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    }
+    return DG_OK;
+}
+
+int dg_batch_get_degree_lanes(const dg_batch* b, int p, int* lanes) {
+    if (!b || !lanes || p < 1 || p > PMAX_GPU) return DG_ERR_INVALID_ARG;
+    *lanes = b->deg_lanes[p] ? b->deg_lanes[p] : b->lanes;
     return DG_OK;
 }
 

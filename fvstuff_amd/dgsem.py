@@ -67,6 +67,8 @@ SIGNATURES = {
     "dg_batch_set_xcd_bins": (C.c_int, [_vp, C.c_int]),
     "dg_batch_set_by_degree": (C.c_int, [_vp, C.c_int]),
     "dg_batch_get_by_degree": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "dg_batch_set_degree_lanes": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "dg_batch_get_degree_lanes": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "dg_batch_get_options": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),
 }
@@ -369,6 +371,15 @@ class Batch:
         v = C.c_int()
         self._check(self.lib.dg_batch_get_by_degree(self._b, C.byref(v)))
         return bool(v.value)
+
+    def set_degree_lanes(self, p, lanes):
+        """By-degree form: lanes per workgroup of degree p's launch (0 = common size)."""
+        self._check(self.lib.dg_batch_set_degree_lanes(self._b, int(p), int(lanes)))
+
+    def degree_lanes(self, p):
+        v = C.c_int()
+        self._check(self.lib.dg_batch_get_degree_lanes(self._b, int(p), C.byref(v)))
+        return v.value
 
     def options(self):
         """(use_graph, lanes, persistent, xcd_bins) currently in use."""

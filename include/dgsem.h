@@ -313,12 +313,19 @@ int         dg_batch_set_xcd_bins(dg_batch* b, int xcd_bins);
  * setting one clears the other.  Results are identical in every form. */
 int         dg_batch_set_by_degree(dg_batch* b, int by_degree);
 int         dg_batch_get_by_degree(const dg_batch* b, int* by_degree);
+/* By-degree form: lanes per workgroup of the launch of degree p (1..7):
+ * 64 / 128 / 256, or 0 to follow dg_batch_set_options (the default); the
+ * getter gives the size in effect for p. */
+int         dg_batch_set_degree_lanes(dg_batch* b, int p, int lanes);
+int         dg_batch_get_degree_lanes(const dg_batch* b, int p, int* lanes);
 /* Current options (any pointer may be NULL). */
 int         dg_batch_get_options(const dg_batch* b, int* use_graph, int* lanes, int* persistent, int* xcd_bins);
 /* Time the three workgroup sizes, each with both orders in the three launch
  * forms (all-degree kernel one workgroup per tile, persistent, by degree), on
  * the given states (they are advanced by a few steps) and keep the fastest;
- * writes the size to *lanes if non-NULL. */
+ * in the by-degree form then refine each degree's workgroup size on its own
+ * (kept if the step runs >= 1 % faster).  Writes the common size to *lanes if
+ * non-NULL. */
 int         dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* lanes);
 
 #ifdef __cplusplus

@@ -204,8 +204,23 @@ def test_batch_by_degree_matches(D, precision):
                 for i, (t, Q) in enumerate(ref):
                     assert ts[i] == t, (key, i)
                     assert np.array_equal(got[i], Q), (key, i)
-    # a common end time inside the run: every instance stops there
+    # each degree's launch with its own workgroup size
     b.set_options(use_graph=True, lanes=128)
+    b.set_by_degree(True)
+    for p, lanes in ((1, 256), (3, 64), (5, 256), (7, 64)):
+        b.set_degree_lanes(p, lanes)
+    assert [b.degree_lanes(p) for p in range(1, 8)] == [256, 128, 64, 128, 256, 128, 64]
+    Qs = b.init_states(ics)
+    ts = b.advance(Qs, 7, cfls)
+    got = host(Qs)
+    for i, (t, Q) in enumerate(ref):
+        assert ts[i] == t and np.array_equal(got[i], Q), ("per-degree lanes", i)
+    for bad in ((0, 64), (8, 64), (3, 96)):
+        with pytest.raises(D.DGError):
+            b.set_degree_lanes(*bad)
+    for p in range(1, 8):
+        b.set_degree_lanes(p, 0)
+    # a common end time inside the run: every instance stops there
     Qs = b.init_states(ics)
     ts = b.advance(Qs, 100000, cfls, t_end=0.01)
     assert all(t == pytest.approx(0.01, abs=1e-13) for t in ts)
