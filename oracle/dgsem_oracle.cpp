@@ -30,6 +30,13 @@
  *                            (theta per element; the AV term through the SBP
  *                            identity W^-1 D^T W D U = W^-1 B U' - U'',
  *                            tests/test_oracle_av.py), free stream
+ *   ICs .................... (tests/test_oracle_ic.py) vortex closed form at
+ *                            the centre and isentropy; Kelvin-Helmholtz closed
+ *                            form at every node (nodes on y = +-1/2 belong to
+ *                            the outer layer, the noise moves v only); the
+ *                            circular Sod front width h/2 recovered from the
+ *                            density (:702-716), plateaus, symmetries; the
+ *                            p = 7 vortex's CFL limit (blow-up at 0.5)
  *   SD sibling ............. (fr_lobatto_2d.cu, tests/test_oracle_sd.py) flux
  *                            nodes = Gauss-Legendre roots and the reference's
  *                            literals, Isf / Dfs exact on polynomials of degree
