@@ -4,6 +4,7 @@
 #include <math.h>
 #include <cmath>
 #include <stdarg.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
@@ -997,6 +998,76 @@ int dg_totals(dg_ctx* c, const double* Q, double out[5]) {
     HIPCK(c, launch_totals(Q, c->cfg.p, c->cfg.K, c->cfg.L, c->cfg.gamma, c->d_part, sums, c->stream));
     HIPCK(c, hipMemcpyAsync(out, sums, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
+    return DG_OK;
+}
+
+// ── checkpoint / restart (no counterpart in the reference) ──────────────────
+// File: a 128-byte header, then the FP64 state (4*K*K*(p+1)^2 doubles in the
+// ABI's layout), both in the host's byte order.  The header records the
+// discretisation the state belongs to, the time and the step count; loading
+// requires the same discretisation as the context (the volume / surface
+// flux, boundaries, shock capturing and precision may differ: they act on
+// the state, they do not define it).
+namespace {
+struct CheckpointHeader {
+    char magic[8];                 // "DGSEMCK" + NUL
+    uint32_t version, header_bytes;   // 1, 128
+    int32_t K, p, scheme, pad0;
+    double x0, y0, L, gamma;
+    double t;
+    int64_t steps, ndoubles;
+    char pad[128 - 8 - 8 - 16 - 40 - 16];
+};
+static_assert(sizeof(CheckpointHeader) == 128, "checkpoint header is 128 bytes");
+const char kCheckpointMagic[8] = {'D', 'G', 'S', 'E', 'M', 'C', 'K', 0};
+}  // namespace
+
+int dg_save_state(dg_ctx* c, const double* Q, double t, long long steps, const char* path) {
+    if (!c || !Q || !path) return fail(c, DG_ERR_INVALID_ARG, "dg_save_state: null argument");
+    CheckpointHeader h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.magic, kCheckpointMagic, 8);
+    h.version = 1;
+    h.header_bytes = sizeof(h);
+    h.K = c->cfg.K; h.p = c->cfg.p; h.scheme = c->cfg.scheme;
+    h.x0 = c->cfg.x0; h.y0 = c->cfg.y0; h.L = c->cfg.L; h.gamma = c->cfg.gamma;
+    h.t = t;
+    h.steps = steps;
+    h.ndoubles = dg_state_size(c);
+    std::vector<double> buf((size_t)h.ndoubles);
+    HIPCK(c, hipMemcpyAsync(buf.data(), Q, buf.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(c, DG_ERR_INVALID_ARG, "dg_save_state: cannot open %s", path);
+    const bool ok = fwrite(&h, sizeof(h), 1, f) == 1 && fwrite(buf.data(), sizeof(double), buf.size(), f) == buf.size();
+    const bool closed = fclose(f) == 0;
+    if (!ok || !closed) return fail(c, DG_ERR_INVALID_ARG, "dg_save_state: cannot write %s", path);
+    return DG_OK;
+}
+
+int dg_load_state(dg_ctx* c, double* Q, double* t, long long* steps, const char* path) {
+    if (!c || !Q || !path) return fail(c, DG_ERR_INVALID_ARG, "dg_load_state: null argument");
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(c, DG_ERR_INVALID_ARG, "dg_load_state: cannot open %s", path);
+    CheckpointHeader h;
+    std::vector<double> buf;
+    const char* why = nullptr;
+    if (fread(&h, sizeof(h), 1, f) != 1) why = "short header";
+    else if (memcmp(h.magic, kCheckpointMagic, 8) || h.version != 1 || h.header_bytes != sizeof(h)) why = "not a checkpoint";
+    else if (h.K != c->cfg.K || h.p != c->cfg.p || h.scheme != c->cfg.scheme || h.x0 != c->cfg.x0 ||
+             h.y0 != c->cfg.y0 || h.L != c->cfg.L || h.gamma != c->cfg.gamma || h.ndoubles != dg_state_size(c))
+        why = "another discretisation";
+    else {
+        buf.resize((size_t)h.ndoubles);
+        if (fread(buf.data(), sizeof(double), buf.size(), f) != buf.size()) why = "short state";
+        else if (fgetc(f) != EOF) why = "trailing bytes";
+    }
+    fclose(f);
+    if (why) return fail(c, DG_ERR_INVALID_ARG, "dg_load_state: %s: %s", path, why);
+    HIPCK(c, hipMemcpyAsync(Q, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));   // the host buffer goes out of scope
+    if (t) *t = h.t;
+    if (steps) *steps = h.steps;
     return DG_OK;
 }
 

@@ -49,6 +49,8 @@ SIGNATURES = {
     "dg_set_graph": (C.c_int, [_vp, C.c_int]),
     "dg_sensor": (C.c_int, [_vp, _vp, _vp]),
     "dg_totals": (C.c_int, [_vp, _vp, _dp]),
+    "dg_save_state": (C.c_int, [_vp, _vp, C.c_double, C.c_longlong, C.c_char_p]),
+    "dg_load_state": (C.c_int, [_vp, _vp, _dp, C.POINTER(C.c_longlong), C.c_char_p]),
     "dg_totals_noalloc": (C.c_int, [C.POINTER(DGConfig), _vp, _vp, _vp]),
     "dg_vortex_exact": (C.c_int, [_vp, C.POINTER(DGICParams), C.c_double, C.c_int, _vp]),
     "dg_error_norms": (C.c_int, [_vp, _vp, _vp, _dp]),
@@ -250,6 +252,18 @@ class Solver:
         s = self.new_elem()
         self._check(self.lib.dg_sensor(self._ctx, _ptr(Q), _ptr(s)))
         return s
+
+    def save_state(self, path, Q, t, steps):
+        """Checkpoint: state Q (device) with its time and step count."""
+        self._check(self.lib.dg_save_state(self._ctx, _ptr(Q), float(t), int(steps), os.fsencode(path)))
+
+    def load_state(self, path, Q=None):
+        """Restart: the checkpoint's state into Q (a new state if None);
+        returns (Q, t, steps)."""
+        Q = self.new_state() if Q is None else Q
+        t, n = C.c_double(), C.c_longlong()
+        self._check(self.lib.dg_load_state(self._ctx, _ptr(Q), C.byref(t), C.byref(n), os.fsencode(path)))
+        return Q, t.value, n.value
 
     def totals(self, Q):
         out = (C.c_double * 5)()

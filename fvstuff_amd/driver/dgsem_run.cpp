@@ -8,12 +8,16 @@
 //
 //   dgsem_run [--case sod|vortex|kh] [--K n] [--p n] [--steps n] [--t-end t]
 //             [--cfl c] [--precision fp64|mixed] [--mode reference|fused]
-//             [--out file]
+//             [--out file] [--checkpoint file] [--restart file]
 //
 // --mode reference issues the reference's separate calls (dg_compute_rhs,
 // dg_cfl_dt, dg_rk3_stage); --mode fused runs the same steps with
 // dg_advance.  Prints one JSON line (steps, t, last dt, status, totals) and,
 // with --out, writes the final state (4*K*K*(p+1)^2 doubles, SIDX order).
+// --restart starts from a checkpoint (dg_load_state: its state, time and
+// step count) instead of the initial condition; --checkpoint writes one at
+// the end (dg_save_state).  The reference has neither: it restarts from
+// t = 0.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -27,7 +31,7 @@
 namespace {
 
 struct Options {
-    std::string ic_case = "sod", precision = "fp64", mode = "reference", out;
+    std::string ic_case = "sod", precision = "fp64", mode = "reference", out, checkpoint, restart;
     int K = 50, p = 3, steps = 100;
     double t_end = 0.0, cfl = 0.6;
 };
@@ -46,6 +50,8 @@ bool parse(int argc, char** argv, Options& o) {
         else if (a == "--precision") o.precision = v;
         else if (a == "--mode") o.mode = v;
         else if (a == "--out") o.out = v;
+        else if (a == "--checkpoint") o.checkpoint = v;
+        else if (a == "--restart") o.restart = v;
         else return false;
     }
     return (o.ic_case == "sod" || o.ic_case == "vortex" || o.ic_case == "kh") &&
@@ -96,7 +102,8 @@ int main(int argc, char** argv) {
     if (!parse(argc, argv, o)) {
         fprintf(stderr,
                 "usage: dgsem_run [--case sod|vortex|kh] [--K n] [--p n] [--steps n] [--t-end t] [--cfl c]\n"
-                "                 [--precision fp64|mixed] [--mode reference|fused] [--out file]\n");
+                "                 [--precision fp64|mixed] [--mode reference|fused] [--out file]\n"
+                "                 [--checkpoint file] [--restart file]\n");
         return 1;
     }
     dg_config cfg;
@@ -120,9 +127,11 @@ int main(int argc, char** argv) {
     HCK(hipMalloc(&d_RHS, bytes));
     HCK(hipMalloc(&d_lam, (size_t)nel * sizeof(double)));
 
-    rc = dg_init_state(ctx, &ic, d_Q);   // ic_kernel + positivity_limiter (:1006-1007)
     double t = 0.0, dt = 0.0;
     int steps = 0;
+    long long steps0 = 0;   // steps of the run a checkpoint was taken from
+    if (o.restart.empty()) rc = dg_init_state(ctx, &ic, d_Q);   // ic_kernel + positivity_limiter (:1006-1007)
+    else rc = dg_load_state(ctx, d_Q, &t, &steps0, o.restart.c_str());
     if (rc == DG_OK && o.mode == "reference") {
         while (steps < o.steps && (o.t_end <= 0.0 || t < o.t_end)) {   // :1022
             double lam_max;
@@ -148,13 +157,15 @@ int main(int argc, char** argv) {
         rc = dg_advance(ctx, d_Q, o.steps, o.cfl, o.t_end, &t, &dt);
         steps = dg_steps_taken(ctx);
     }
+    if (rc == DG_OK && !o.checkpoint.empty()) rc = dg_save_state(ctx, d_Q, t, steps0 + steps, o.checkpoint.c_str());
     double tot[5] = {0, 0, 0, 0, 0};
     if (rc == DG_OK) rc = dg_totals(ctx, d_Q, tot);
     if (rc != DG_OK) fprintf(stderr, "status %d (%s): %s\n", rc, dg_status_string(rc), dg_last_error(ctx));
     printf("{\"case\": \"%s\", \"mode\": \"%s\", \"precision\": \"%s\", \"K\": %d, \"p\": %d, \"steps\": %d, "
-           "\"t\": %.17g, \"dt\": %.17g, \"status\": %d, \"totals\": [%.17g, %.17g, %.17g, %.17g, %.17g]}\n",
-           o.ic_case.c_str(), o.mode.c_str(), o.precision.c_str(), o.K, o.p, steps, t, dt, rc, tot[0], tot[1],
-           tot[2], tot[3], tot[4]);
+           "\"total_steps\": %lld, \"t\": %.17g, \"dt\": %.17g, \"status\": %d, "
+           "\"totals\": [%.17g, %.17g, %.17g, %.17g, %.17g]}\n",
+           o.ic_case.c_str(), o.mode.c_str(), o.precision.c_str(), o.K, o.p, steps, steps0 + steps, t, dt, rc,
+           tot[0], tot[1], tot[2], tot[3], tot[4]);
     if (!o.out.empty()) {
         std::vector<double> h((size_t)n);
         HCK(hipMemcpy(h.data(), d_Q, bytes, hipMemcpyDeviceToHost));

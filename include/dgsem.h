@@ -265,6 +265,19 @@ int         dg_frame_fields(dg_ctx* ctx, const double* Q, double* rho_el, double
  * picks a heuristic schedule.  Writes the chosen values if pointers non-NULL. */
 int         dg_autotune(dg_ctx* ctx, int* elems_per_block, int* xcd_remap);
 
+/* ── Checkpoint / restart (no counterpart in the reference, whose frames are
+ * lossy PPM images): dg_save_state writes device state Q with its time and
+ * step count to `path` (a 128-byte header -- magic "DGSEMCK", version 1, K,
+ * p, scheme, x0, y0, L, gamma, t, steps, the number of doubles -- then the
+ * FP64 state in the layout above, host byte order); dg_load_state reads such
+ * a file into device state Q (t, steps may be NULL).  Loading requires the
+ * file's discretisation (K, p, scheme, domain, gamma) to be the context's,
+ * else DG_ERR_INVALID_ARG with the reason in dg_last_error.  A run resumed
+ * from a checkpoint takes the same steps, bit for bit, as one that continues
+ * with the same state and time in a new dg_advance call. */
+int         dg_save_state(dg_ctx* ctx, const double* Q, double t, long long steps, const char* path);
+int         dg_load_state(dg_ctx* ctx, double* Q, double* t, long long* steps, const char* path);
+
 /* ── Batched independent solves (no counterpart in the reference, which runs
  * one solve per process): ensembles of instances that differ in K, p, domain,
  * IC and CFL but share the flux / boundary / precision switches.  Each stage

@@ -56,6 +56,9 @@ def test_host_only_calls():
     ic = capi.DGICParams()
     lib.dg_ic_default(C.byref(ic), 1)
     assert ic.kind == 1 and ic.vortex_eps == 5.0 and ic.xc == 5.0 and ic.u0 == 1.0
+    # checkpoint calls reject null arguments before touching a device
+    assert lib.dg_save_state(None, None, 0.0, 0, b"x") == capi.DG_ERR_INVALID_ARG
+    assert lib.dg_load_state(None, None, None, None, b"x") == capi.DG_ERR_INVALID_ARG
 
 
 def test_struct_layout_matches_header(tmp_path):
