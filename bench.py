@@ -32,9 +32,11 @@ HBM_PEAK_GBS = 8000.0
 # sources' is not reported (the kernel changed since it was measured).
 PMC_TRAFFIC = {(512, 7, "mixed"): (1.437e9, "profiles/stage_kernel_p7_vortex512.md", "749016932c8ec24c")}
 FP64_PEAK_TFLOPS = 78.6
-# the sources the stage kernels are compiled from
+# the sources the stage kernels are compiled from, and the build script that
+# holds the compiler flags
 KERNEL_SOURCES = ("fvstuff_amd/csrc/dgsem_kernels.hpp", "fvstuff_amd/csrc/dgsem_physics.hpp",
-                  "fvstuff_amd/csrc/dgsem_inst.hip", "fvstuff_amd/csrc/dgsem_launch.hpp")
+                  "fvstuff_amd/csrc/dgsem_inst.hip", "fvstuff_amd/csrc/dgsem_launch.hpp",
+                  "fvstuff_amd/build.py")
 # SSP-RK3 steps of the accuracy comparison: the headline configuration's count
 ACCURACY_STEPS = 200
 

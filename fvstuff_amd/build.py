@@ -30,6 +30,12 @@ FLAGS = [
     # inlined copy of a helper rounds alike (a node converted in two places
     # gives the same bits, which keeps results independent of the tiling)
     "-ffp-contract=on",
+    # LLVM's memory-clause scheduling strategy: the shock-capturing FP32
+    # kernel at p = 3 takes 96 instead of 122 VGPRs (5 waves per SIMD instead
+    # of 4) and runs 1.1-1.7 % faster (KH 1024^2 mixed); vortex p = 7, the
+    # batched ensemble and the SD sibling within +-0.5 % (FP64 SD 1 % faster);
+    # the same arithmetic, so the same bits.  (max-ilp: FP64 KH 21 % slower.)
+    "-mllvm", "-amdgpu-sched-strategy=max-memory-clause",
     "-Wno-unused-command-line-argument",
 ]
 
