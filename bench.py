@@ -30,7 +30,17 @@ HBM_PEAK_GBS = 8000.0
 # (bytes, profile that lists the runs and the scaling, KERNEL_SOURCES hash of
 # the build that was profiled).  A figure whose hash differs from the current
 # sources' is not reported (the kernel changed since it was measured).
-PMC_TRAFFIC = {(512, 7, "mixed"): (1.437e9, "profiles/stage_kernel_p7_vortex512.md", "749016932c8ec24c")}
+# Keyed (K, p, form, elements per workgroup, XCD order): the tile order sets
+# how many face-neighbour reads hit another XCD's L2.
+_PROF = "profiles/stage_kernel_p7_vortex512.md"
+_SRC = "6c35d77be106bb5d"
+PMC_TRAFFIC = {
+    (512, 7, "mixed", 16, 1): (1.43694e9, _PROF, _SRC), (512, 7, "mixed", 16, 0): (1.50037e9, _PROF, _SRC),
+    (512, 7, "mixed", 8, 1): (1.43694e9, _PROF, _SRC), (512, 7, "mixed", 8, 0): (1.56749e9, _PROF, _SRC),
+    (512, 7, "mixed", 4, 1): (1.43693e9, _PROF, _SRC), (512, 7, "mixed", 4, 0): (1.70170e9, _PROF, _SRC),
+    (512, 7, "fp64", 8, 1): (1.43748e9, _PROF, _SRC), (512, 7, "fp64", 8, 0): (1.56799e9, _PROF, _SRC),
+    (512, 7, "fp64", 4, 1): (1.43744e9, _PROF, _SRC), (512, 7, "fp64", 4, 0): (1.70228e9, _PROF, _SRC),
+}
 FP64_PEAK_TFLOPS = 78.6
 # the sources the stage kernels are compiled from, and the build script that
 # holds the compiler flags
@@ -531,7 +541,9 @@ def main():
     log(f"[bench] HBM reference: {ref}")
 
     src_hash = kernel_sources_hash()
-    traffic = PMC_TRAFFIC.get((K, p, main_form))
+    sched_used = res[main_form]["tuned"] or (tuple(int(v) for v in args.schedule.split(",")) if args.schedule
+                                              else None)
+    traffic = PMC_TRAFFIC.get((K, p, main_form) + tuple(sched_used)) if sched_used else None
     traffic_ok = traffic is not None and traffic[2] == src_hash
     out = {
         "metric": "DOF*stage updates/s, mixed vs FP64 DGSEM at equal L2 error; % HBM peak",
@@ -554,8 +566,10 @@ def main():
         "accuracy": acc,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic[0] if traffic_ok else None,
-                     "traffic_source": (f"{traffic[1]} (kernel sources {traffic[2]})" if traffic_ok else
-                                        f"not measured for kernel sources {src_hash}"),
+                     "traffic_source": (f"{traffic[1]} (kernel sources {traffic[2]}, schedule {sched_used})"
+                                        if traffic_ok else
+                                        f"not measured for kernel sources {src_hash}, schedule {sched_used}"),
+                     "schedule": sched_used,
                      "kernel": f"stage_kernel<{p},{'float' if main_form == 'mixed' else 'double'},lean,Ranocha>",
                      "kernel_sources": src_hash,
                      "bytes_per_dof_stage": b_alg, "avg_launch_ms": ms_launch,
