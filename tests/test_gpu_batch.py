@@ -244,3 +244,28 @@ def test_batch_forms_exclude_each_other(D):
     assert out[0][0] == out[1][0]
     for a, c in zip(out[0][1], out[1][1]):
         assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_batch_by_degree_tiny_instances(D, precision):
+    """Instances of 1, 2 and 3 elements per side at every degree (launches of
+    fewer tiles than there are XCDs, single-element periodic domains whose
+    faces are their own) in the by-degree form with the XCD-aware order:
+    every instance gets its single solve's bits."""
+    cfgs = [vortex_config(K=K, p=p, precision=precision) for p in range(1, 8) for K in (1, 2, 3)]
+    ics = [make_ic("vortex") for _ in cfgs]
+    cfls = [0.2] * len(cfgs)
+    b = D.Batch(cfgs)
+    b.set_by_degree(True)
+    b.set_xcd_bins(True)
+    for lanes in (64, 256):
+        b.set_options(use_graph=True, lanes=lanes)
+        Qs = b.init_states(ics)
+        ts = b.advance(Qs, 5, cfls)
+        got = host(Qs)
+        for i, (cfg, ic) in enumerate(zip(cfgs, ics)):
+            s = D.Solver(cfg)
+            Q = s.init_state(ic)
+            t, _ = s.advance(Q, 5, 0.2)
+            assert ts[i] == t, (lanes, cfg.K, cfg.p)
+            assert np.array_equal(host([Q])[0], got[i]), (lanes, cfg.K, cfg.p)
