@@ -136,45 +136,52 @@ typedef const int2 ConstInt2;
 // (The degree is a template argument: every object file instantiates this
 // kernel, and instantiations that differ only in DG_P would share one
 // symbol, of which the link keeps one.)
-template <int P, typename Real, int VOL>
-__global__ __launch_bounds__(BLOCK, (Occ<P, Real, VOL, false>::waves)) void batch_degree_kernel(
+// FULL: the shock-capturing body (AV and the Zhang-Shu limiter), as the
+// single-solve stage_kernel<P, Real, true, VOL> runs it.
+template <int P, typename Real, bool FULL, int VOL>
+__global__ __launch_bounds__(BLOCK, (Occ<P, Real, VOL, FULL>::waves)) void batch_degree_kernel(
     const KArgs* __restrict__ args, const int2* __restrict__ tiles, int ntiles, int stage, StepCtl sc, int xcd) {
     const int t = tile_of_block(blockIdx.x, gridDim.x, xcd);
     if (t >= ntiles) return;
     const int2 tl = ((ConstInt2*)tiles)[t];
     const KArgs a = ((ConstKArgs*)args)[(size_t)tl.x * 3 + (stage - 1)];
-    stage_tile<P, Real, false, VOL, (sizeof(Real) == 8), true, true, true, false>(a, sc, tl.y, tl.y == 0,
+    stage_tile<P, Real, FULL, VOL, (sizeof(Real) == 8), true, !FULL, true, false>(a, sc, tl.y, tl.y == 0,
                                                                                     (int)threadIdx.x);
 }
 
-template <typename Real, int VOL>
+template <typename Real, bool FULL, int VOL>
 static hipError_t launch_degree_one(const KArgs* args, const int2* tiles, int ntiles, int stage, StepCtl sc,
                                     int block, size_t lds, int xcd, hipStream_t s) {
     static size_t done = 0;
-    hipError_t e = set_lds(batch_degree_kernel<DG_P, Real, VOL>, lds, done);
+    hipError_t e = set_lds(batch_degree_kernel<DG_P, Real, FULL, VOL>, lds, done);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((batch_degree_kernel<DG_P, Real, VOL>), dim3(ntiles), dim3(block), lds, s, args, tiles,
-                       ntiles, stage, sc, xcd);
+    hipLaunchKernelGGL((batch_degree_kernel<DG_P, Real, FULL, VOL>), dim3(ntiles), dim3(block), lds, s, args,
+                       tiles, ntiles, stage, sc, xcd);
     return hipGetLastError();
 }
 
-template <typename Real>
+template <typename Real, bool FULL>
 static hipError_t launch_degree_vol(const KArgs* args, const int2* tiles, int ntiles, int stage, StepCtl sc,
                                     int vol, int block, size_t lds, int xcd, hipStream_t s) {
     switch (vol) {
-        case VOL_RANOCHA: return launch_degree_one<Real, VOL_RANOCHA>(args, tiles, ntiles, stage, sc, block, lds, xcd, s);
+        case VOL_RANOCHA:
+            return launch_degree_one<Real, FULL, VOL_RANOCHA>(args, tiles, ntiles, stage, sc, block, lds, xcd, s);
         case VOL_CHANDRASHEKAR:
-            return launch_degree_one<Real, VOL_CHANDRASHEKAR>(args, tiles, ntiles, stage, sc, block, lds, xcd, s);
-        default: return launch_degree_one<Real, VOL_COLLOCATION>(args, tiles, ntiles, stage, sc, block, lds, xcd, s);
+            return launch_degree_one<Real, FULL, VOL_CHANDRASHEKAR>(args, tiles, ntiles, stage, sc, block, lds, xcd, s);
+        default:
+            return launch_degree_one<Real, FULL, VOL_COLLOCATION>(args, tiles, ntiles, stage, sc, block, lds, xcd, s);
     }
 }
 
 hipError_t DG_CAT(launch_batch_degree_p, DG_P)(const KArgs* args, const int2* tiles, int ntiles, int stage,
-                                               StepCtl sc, bool fp32, int vol, int block, size_t lds, int xcd,
-                                               hipStream_t s) {
+                                               StepCtl sc, bool fp32, bool full, int vol, int block, size_t lds,
+                                               int xcd, hipStream_t s) {
     if (ntiles < 1) return hipSuccess;
-    return fp32 ? launch_degree_vol<float>(args, tiles, ntiles, stage, sc, vol, block, lds, xcd, s)
-                : launch_degree_vol<double>(args, tiles, ntiles, stage, sc, vol, block, lds, xcd, s);
+    if (fp32)
+        return full ? launch_degree_vol<float, true>(args, tiles, ntiles, stage, sc, vol, block, lds, xcd, s)
+                    : launch_degree_vol<float, false>(args, tiles, ntiles, stage, sc, vol, block, lds, xcd, s);
+    return full ? launch_degree_vol<double, true>(args, tiles, ntiles, stage, sc, vol, block, lds, xcd, s)
+                : launch_degree_vol<double, false>(args, tiles, ntiles, stage, sc, vol, block, lds, xcd, s);
 }
 
 hipError_t DG_CAT(launch_limit_p, DG_P)(double* Q, const double* w64, int NEL, int E, double g, double eps_rho,

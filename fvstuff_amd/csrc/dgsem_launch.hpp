@@ -18,17 +18,18 @@ typedef hipError_t (*SensorLauncher)(const double* Q, double* sensor, const doub
 
 // Batched stage of one degree: the tiles of every instance of that degree in
 // one launch of `ntiles` workgroups of `block` threads, with the single-solve
-// kernel's body and schedule; xcd: XCD-aware block-to-tile order.
+// kernel's body and schedule (full: the shock-capturing body); xcd:
+// XCD-aware block-to-tile order.
 typedef hipError_t (*BatchDegreeLauncher)(const KArgs* args, const int2* tiles, int ntiles, int stage,
-                                          StepCtl sc, bool fp32, int vol, int block, size_t lds, int xcd,
-                                          hipStream_t s);
+                                          StepCtl sc, bool fp32, bool full, int vol, int block, size_t lds,
+                                          int xcd, hipStream_t s);
 
 #define DG_DECLARE_P(P)                                                                              \
     hipError_t launch_stage_p##P(const KArgs&, bool, bool, bool, int, int, size_t, hipStream_t);    \
     hipError_t launch_limit_p##P(double*, const double*, int, int, double, double, double, hipStream_t); \
     hipError_t launch_sensor_p##P(const double*, double*, const double*, int, double, double, double, hipStream_t); \
-    hipError_t launch_batch_degree_p##P(const KArgs*, const int2*, int, int, StepCtl, bool, int, int, size_t, int, \
-                                        hipStream_t);
+    hipError_t launch_batch_degree_p##P(const KArgs*, const int2*, int, int, StepCtl, bool, bool, int, int, size_t, \
+                                        int, hipStream_t);
 DG_DECLARE_P(1)
 DG_DECLARE_P(2)
 DG_DECLARE_P(3)

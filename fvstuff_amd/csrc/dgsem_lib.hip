@@ -1263,6 +1263,9 @@ struct dg_batch {
     };
     bool by_degree = false;
     std::vector<DegreeRun> runs;
+    // AV and / or the limiter on (every instance alike): the shock-capturing
+    // stage kernels, which run in the by-degree form only
+    bool full = false;
     int deg_lanes[PMAX_GPU + 1] = {0};  // by-degree form: lanes per workgroup of degree p (0 = lanes)
     hipGraphExec_t gexec = nullptr;
     char err[512] = {0};
@@ -1289,8 +1292,8 @@ static hipError_t batch_launch_stage(dg_batch* b, int stage, StepCtl sc) {
     if (b->by_degree) {
         for (const dg_batch::DegreeRun& r : b->runs) {
             hipError_t e = batch_degree_launcher(r.p)(b->d_args, b->d_tiles + r.first, r.count, stage, sc, fp32,
-                                                      b->common.volume_flux, r.block, r.lds, b->xcd_bins ? 1 : 0,
-                                                      b->cap);
+                                                      b->full, b->common.volume_flux, r.block, r.lds,
+                                                      b->xcd_bins ? 1 : 0, b->cap);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -1329,7 +1332,7 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
         const dg_config& f = c->cfg;
         const bool fp32 = f.precision == DG_MIXED;
         const int E = batch_elems_per_block(c->N, batch_lanes_of(b, c->cfg.p));
-        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(b), false, false, false);
+        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(b), f.av != 0, false, f.limiter != 0);
         for (int s = 1; s <= 3; s++) {
             KArgs a;
             memset(&a, 0, sizeof(a));
@@ -1342,7 +1345,7 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
             a.st = c->d_st;
             a.P = f.p; a.K = f.K; a.NEL = c->NEL; a.E = E; a.xcd = 0;
             a.mode = MODE_STAGE; a.stage = s; a.vol = f.volume_flux; a.surf = f.surface_flux; a.bc = f.bc;
-            a.av = 0; a.limiter = 0;
+            a.av = f.av; a.limiter = f.limiter;
             a.h = c->h; a.gamma = f.gamma;
             a.ca = s == 1 ? 0.0 : (s == 2 ? 0.75 : 1.0 / 3.0);
             a.cb = s == 1 ? 1.0 : (s == 2 ? 0.25 : 2.0 / 3.0);
@@ -1382,7 +1385,7 @@ static hipError_t batch_tiles(dg_batch* b) {
         const int E = batch_elems_per_block(c->N, batch_lanes_of(b, c->cfg.p));
         ntl[i] = (c->NEL + E - 1) / E;
         tile_cost[i] = (double)E * c->N * c->N * c->N;
-        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(b), false, false, false);
+        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(b), c->cfg.av != 0, false, c->cfg.limiter != 0);
         lds = std::max(lds, (size_t)L.total);
     }
     // instances by tile cost, heaviest first (stable: ties keep their order)
@@ -1399,7 +1402,8 @@ static hipError_t batch_tiles(dg_batch* b) {
                 for (int t = 0; t < ntl[i]; t++) tiles.push_back(make_int2(i, t));
                 const int E = batch_elems_per_block(c->N, batch_lanes_of(b, c->cfg.p));
                 r.block = std::min(BLOCK, (E * 2 * c->N + 63) / 64 * 64);
-                r.lds = (size_t)lds_layout(c->N, E, fp32 ? 4 : 8, true, false, false, false).total;
+                r.lds = (size_t)lds_layout(c->N, E, fp32 ? 4 : 8, true, c->cfg.av != 0, false,
+                                           c->cfg.limiter != 0).total;
             }
             r.count = (int)tiles.size() - r.first;
             if (r.count > 0) b->runs.push_back(r);
@@ -1492,11 +1496,14 @@ int dg_batch_create(const dg_config* cfgs, int n, void* stream, dg_batch** out) 
     for (int i = 0; i < n; i++) {
         const dg_config& f = cfgs[i];
         if (f.volume_flux != c0.volume_flux || f.surface_flux != c0.surface_flux || f.bc != c0.bc ||
-            f.precision != c0.precision || f.av || f.limiter || f.scheme != DG_SCHEME_DGSEM)
+            f.precision != c0.precision || (f.av != 0) != (c0.av != 0) || (f.limiter != 0) != (c0.limiter != 0) ||
+            f.scheme != DG_SCHEME_DGSEM)
             return DG_ERR_INVALID_ARG;
     }
     dg_batch* b = new dg_batch();
     b->n = n;
+    b->full = c0.av || c0.limiter;
+    b->by_degree = b->full;
     b->stream = (hipStream_t)stream;
     b->common = c0;
     b->ctx.assign(n, nullptr);
@@ -1567,6 +1574,8 @@ int dg_batch_set_xcd_bins(dg_batch* b, int xcd_bins);
 
 int dg_batch_set_persistent(dg_batch* b, int persistent) {
     if (!b) return DG_ERR_INVALID_ARG;
+    if (persistent && b->full)
+        return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_set_persistent: AV / limiter run in the by-degree form only");
     if (persistent && b->by_degree) {   // the persistent form is the all-degree kernel's
         b->by_degree = false;
         BHIPCK(b, batch_tiles(b));
@@ -1665,6 +1674,7 @@ int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* la
     // kernel with one workgroup per tile or persistent, one launch per degree)
     for (int cand = 0; cand < 18 && !rc; cand++) {
         const int l = cands[cand / 6], form = (cand % 6) >> 1, bins = cand & 1;
+        if (b->full && form != 2) continue;   // the shock-capturing kernels: by degree only
         if ((rc = dg_batch_set_options(b, b->use_graph, l))) break;
         if ((rc = dg_batch_set_by_degree(b, form == 2))) break;
         if ((rc = dg_batch_set_xcd_bins(b, bins))) break;
@@ -1717,6 +1727,8 @@ int dg_batch_get_options(const dg_batch* b, int* use_graph, int* lanes, int* per
 
 int dg_batch_set_by_degree(dg_batch* b, int by_degree) {
     if (!b) return DG_ERR_INVALID_ARG;
+    if (!by_degree && b->full)
+        return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_set_by_degree: AV / limiter run in the by-degree form only");
     if ((by_degree != 0) == b->by_degree) return DG_OK;
     b->by_degree = by_degree != 0;
     if (b->by_degree) b->persistent = false;

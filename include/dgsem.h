@@ -286,8 +286,12 @@ int         dg_load_state(dg_ctx* ctx, double* Q, double* t, long long* steps, c
  * by-degree form, one launch per degree present (the instances of that
  * degree, each with the single-solve kernel of that degree); each instance
  * keeps its own device-side dt and time.  The stage launches of the time loop
- * are replayed from a captured HIP graph.  AV and the limiter are not
- * available in batched mode (DG_ERR_INVALID_ARG). */
+ * are replayed from a captured HIP graph.  AV and the limiter must be on in
+ * every instance or in none (DG_ERR_INVALID_ARG otherwise); such a batch runs
+ * in the by-degree form only, with the single-solve shock-capturing kernel of
+ * each degree, and refuses dg_batch_set_by_degree(b, 0) and
+ * dg_batch_set_persistent(b, 1).  Their parameters (av_cav, av_ksensor,
+ * eps_rho, eps_p) may differ between instances. */
 typedef struct dg_batch dg_batch;
 
 int         dg_batch_create(const dg_config* cfgs, int n, void* stream, dg_batch** out);
@@ -322,7 +326,8 @@ int         dg_batch_set_xcd_bins(dg_batch* b, int xcd_bins);
  * highest degree first), each launch running the single-solve lean kernel of
  * its degree with its own register budget and schedule; xcd_bins then selects
  * the XCD-aware block order within each launch.  0 = the all-degree kernel
- * (the default).  The by-degree and persistent forms exclude each other:
+ * (the default without AV / limiter; a batch with them is always by degree).
+ * The by-degree and persistent forms exclude each other:
  * setting one clears the other.  Results are identical in every form. */
 int         dg_batch_set_by_degree(dg_batch* b, int by_degree);
 int         dg_batch_get_by_degree(const dg_batch* b, int* by_degree);
