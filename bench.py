@@ -106,6 +106,8 @@ def parse():
     ap.add_argument("--xcd-bins", type=int, default=-1, choices=[-1, 0, 1],
                     help="batched: tile list in per-XCD bins (1), heaviest first (0), tuned (-1)")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--ensemble", default="vortex", choices=["vortex", "sod"],
+                    help="batched: vortex instances, or the circular-Sod setup with AV and limiter")
     return ap.parse_args()
 
 
@@ -373,15 +375,19 @@ def bench_sd(args):
 
 
 def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, xcd_bins=-1, autotune=True,
-                by_degree=-1):
-    """Seeded ensemble of isentropic-vortex instances (K ~ U{16..128},
-    p ~ U{3,5,7}, strength ~ U[1,5], Mach ~ U[0.1,0.8]) stepped in one
-    launch per stage for all instances (or one per degree present) vs
-    instance by instance (wall clock around each, after a synchronisation)."""
+                by_degree=-1, kind="vortex"):
+    """Seeded ensemble stepped in one launch per stage for all instances (or
+    one per degree present) vs instance by instance (wall clock around each,
+    after a synchronisation).  kind "vortex": isentropic-vortex instances
+    (K ~ U{16..128}, p ~ U{3,5,7}, strength ~ U[1,5], Mach ~ U[0.1,0.8]);
+    "sod": the reference's circular-Sod setup with AV and the limiter
+    (K ~ U{16..128}, p ~ U{3,5,7}, inner density ~ U[2,11], pressure
+    ~ U[2,10]), which runs one launch per degree only."""
     import torch
     from fvstuff_amd import dgsem as D
-    from fvstuff_amd.capi import ensemble
-    cfgs, ics, cfls = ensemble(n=instances, seed=seed, precision=form)
+    from fvstuff_amd.capi import ensemble, sod_ensemble
+    make = sod_ensemble if kind == "sod" else ensemble
+    cfgs, ics, cfls = make(n=instances, seed=seed, precision=form)
     b = D.Batch(cfgs)
     Qs = b.init_states(ics)
     tuned = autotune and not (lanes or persistent >= 0 or xcd_bins >= 0 or by_degree >= 0)
@@ -389,7 +395,8 @@ def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, xc
         b.autotune(Qs, cfls)
     else:
         b.set_options(use_graph=True, lanes=lanes)
-        b.set_by_degree(max(0, by_degree))
+        if by_degree >= 0 or kind != "sod":
+            b.set_by_degree(max(0, by_degree))
         b.set_xcd_bins(max(0, xcd_bins))
         if persistent > 0:
             b.set_persistent(True)
@@ -409,7 +416,8 @@ def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, xc
         wall = time.perf_counter() - t0
         res[mode] = {"value": b.dofs * 3 * steps / wall, "ms_per_step": wall * 1e3 / steps}
         log(f"[bench-batched] {form} {mode}: {res[mode]}")
-    return {"workload": f"{instances} vortex instances (seed {seed}), {form}", "unit": "DOF*stage/s",
+    what = "circular-Sod instances with AV and limiter" if kind == "sod" else "vortex instances"
+    return {"workload": f"{instances} {what} (seed {seed}), {form}", "unit": "DOF*stage/s",
             "dof": b.dofs, "steps": steps, "lanes_per_workgroup": lanes, "persistent": persistent,
             "xcd_bins": xcd_bins, "by_degree": by_degree, "lanes_by_degree": lanes_by_degree, "tuned": tuned,
             "batched": res["batched"], "one_by_one": res["each"],
@@ -419,7 +427,8 @@ def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, xc
 def bench_batched(args):
     print(json.dumps(run_batched(args.instances, args.seed, args.forms.split(",")[-1], args.steps, args.warmup,
                                  lanes=args.lanes, persistent=args.persistent, xcd_bins=args.xcd_bins,
-                                 autotune=not args.no_autotune, by_degree=args.by_degree)))
+                                 autotune=not args.no_autotune, by_degree=args.by_degree,
+                                 kind=args.ensemble)))
 
 
 def bench_copy(args):

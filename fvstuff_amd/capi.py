@@ -114,6 +114,31 @@ def kh_config(K=64, p=3, precision="mixed", **kw):
     return make_config(**args)
 
 
+def sod_ensemble(n=256, seed=2024, precision="mixed", k_range=(16, 128), degrees=(3, 5, 7),
+                 rho_range=(2.0, 11.0), p_range=(2.0, 10.0)):
+    """Seed-drawn instances of the reference's default configuration
+    (dgsem_lobatto_2d.cu:61-126, :915-922: circular Sod on the unit square
+    with walls, Chandrashekar volume flux, HLLC faces, Ducros AV, Zhang-Shu
+    limiter): K ~ U{k_range}, p ~ U{degrees}, inner density ~ U[rho_range]
+    and pressure ~ U[p_range] (outer state 1, 1).  Returns (configs, ics,
+    cfls).  The reference's CFL 0.6 holds at p = 3 only: in the FP64 oracle,
+    K = 16 Sod blows up within 300 steps at CFL 0.6 for p = 5 and at 0.4 for
+    p = 7; 0.5 (p <= 3), 0.3 (p = 5) and 0.2 (p = 7) run 500 steps at
+    K = 16 and 32 for both ends of the ratio ranges."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    cfgs, ics, cfls = [], [], []
+    for _ in range(n):
+        K = int(rng.integers(k_range[0], k_range[1] + 1))
+        p = int(rng.choice(degrees))
+        rho_hi = float(rng.uniform(*rho_range))
+        p_hi = float(rng.uniform(*p_range))
+        cfgs.append(make_config(K=K, p=p, precision=precision))
+        ics.append(make_ic("sod", rho_hi=rho_hi, p_hi=p_hi))
+        cfls.append(0.5 if p <= 3 else (0.3 if p <= 5 else 0.2))
+    return cfgs, ics, cfls
+
+
 def ensemble(n=256, seed=2024, precision="mixed", k_range=(16, 128), degrees=(3, 5, 7),
              eps_range=(1.0, 5.0), mach_range=(0.1, 0.8), gamma=1.4):
     """Seed-drawn isentropic-vortex instances: K ~ U{k_range}, p ~ U{degrees},

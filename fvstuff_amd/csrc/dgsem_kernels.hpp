@@ -84,6 +84,27 @@ struct KArgs {
     int off_U, off_av, off_lam, off_el;   // byte offsets of the LDS regions
 };
 
+// Read-only operator table (D, weights, Lav, SD matrices) read through the
+// constant address space: a uniform index then compiles to a scalar load.
+// Read as plain global memory, every table read after an `asm volatile`
+// (the accumulator pins, which the compiler must assume may write memory)
+// or a global store became a per-lane vector load (FP64 p = 7: 56 per line
+// thread in the volume loop, 120 with the AV).  Divergent indices (the MFMA
+// form's Lav rows) still compile to vector loads.  The tables are written
+// once at set-up and never by a kernel.
+template <typename Real>
+struct CTab {
+    const Real* p;
+    __device__ __forceinline__ Real operator[](int i) const {
+#ifdef __HIP_DEVICE_COMPILE__
+        return ((const __attribute__((address_space(4))) Real*)p)[i];
+#else
+        return p[i];
+#endif
+    }
+    __device__ __forceinline__ CTab operator+(int o) const { return CTab{p + o}; }
+};
+
 // Host side: fill the derived constants of an argument block whose h, gamma,
 // K and P are set (the same IEEE divisions the device would do).
 inline void set_derived(KArgs& a) {
@@ -415,7 +436,7 @@ __device__ __forceinline__ void pin_pair(Real (&acc)[N][4], int i, int k) {
 // acc[N-1] hold their first terms already; node k of 0 < k < N-1 receives its
 // first term from the pair (0, k), which initialises it.
 template <int N, typename Real, int VOL, bool PIN, typename PH>
-__device__ __forceinline__ void line_pairs(const PH& ph, const Prim<Real> (&hw)[N], const Real* Dv,
+__device__ __forceinline__ void line_pairs(const PH& ph, const Prim<Real> (&hw)[N], CTab<Real> Dv,
                                            Real (&acc)[N][4]) {
     Real c[4] = {0, 0, 0, 0};
     if (VOL == VOL_RANOCHA) {
@@ -444,7 +465,7 @@ __device__ __forceinline__ void line_pairs(const PH& ph, const Prim<Real> (&hw)[
 // ── volume contribution of one line plus its two face terms:
 //    acc[i] = sum_k Dv_ik F#(U_i,U_k) + [i = 0] fm + [i = N-1] fp ──────────
 template <int N, typename Real, int VOL, bool PIN, typename PH>
-__device__ __forceinline__ void line_volume(const PH& ph, const Prim<Real> (&hw)[N], const Real* Dv,
+__device__ __forceinline__ void line_volume(const PH& ph, const Prim<Real> (&hw)[N], CTab<Real> Dv,
                                             const Real (&fm)[4], const Real (&fp)[4], Real (&acc)[N][4]) {
     if (VOL == VOL_COLLOCATION) {
         // linear contraction sum_k D_ik f(U_k) (:431-439)
@@ -488,7 +509,7 @@ __device__ __forceinline__ void line_volume(const PH& ph, const Prim<Real> (&hw)
 //    y-lines read the momenta swapped and run the x-direction code. ──
 template <int N, typename Real, typename PH>
 __device__ __forceinline__ void sd_line(const PH& ph, const double* eU, int lbase, int lstride, bool ydir,
-                                        const Real* Isf, const Real* Dsd, const Real (&fm)[4],
+                                        CTab<Real> Isf, CTab<Real> Dsd, const Real (&fm)[4],
                                         const Real (&fp)[4], Real (&acc)[N][4]) {
     constexpr int N2 = N * N, NF = N + 1;
     const int q1 = ydir ? 2 : 1, q2 = ydir ? 1 : 2;
@@ -585,9 +606,9 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     const double g = a.gamma;
     const Gas64 gs{g, a.gm1, a.efloor};
 
-    const Real* D = (const Real*)a.tab;
-    const Real* winv = D + N * N + N;
-    const Real* Dv = D + N * N + 2 * N;             // volume matrix
+    const CTab<Real> D{(const Real*)a.tab};
+    const CTab<Real> winv = D + (N * N + N);
+    const CTab<Real> Dv = D + (N * N + 2 * N);      // volume matrix
     // the tile's first node in every FP64 stream (variable q at + q*S)
     const size_t tile_off = (size_t)el0 * N2;
 
@@ -930,7 +951,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             // owns the 8 elements of its lines, and its reads precede its
             // writes).  Elements past the tile hold the stand-in state.
             typedef float f4 __attribute__((ext_vector_type(4)));
-            const Real* Lav = D + 2 * N * N + 2 * N;
+            const CTab<Real> Lav = D + (2 * N * N + 2 * N);
             const int l = tid & 63, c = l & 3, q = (l >> 2) & 3;
             float lav[4];
 #pragma unroll
@@ -1034,8 +1055,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         }
 
         if (SD) {
-            const Real* Isf = D + 3 * N * N + 2 * N;     // [N+1][N]
-            const Real* Dsd = Isf + (N + 1) * N;         // [N][N+1]
+            const CTab<Real> Isf = D + (3 * N * N + 2 * N);     // [N+1][N]
+            const CTab<Real> Dsd = Isf + (N + 1) * N;           // [N][N+1]
             sd_line<N, Real>(ph, sU + le * 4 * N2, lbase, lstride, ydir, Isf, Dsd, cm, cp, acc);
         } else {
             line_volume<N, Real, VOL, PIN>(ph, hw, Dv, cm, cp, acc);
@@ -1045,7 +1066,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         //    = -nu jac^2 Lav Uc, Lav = W^-1 D^T W D precomputed (one pass) ──
         if (FULL && !SD && !MFMA_AV && a.av) {
             const Real f = -sNu[le] * jac * jac;
-            const Real* Lav = D + 2 * N * N + 2 * N;
+            const CTab<Real> Lav = D + (2 * N * N + 2 * N);
             Real uc[N][4];      // conserved variables of the line's nodes (u, v as rotated)
 #pragma unroll
             for (int k = 0; k < N; k++) {

@@ -1,8 +1,9 @@
-"""The seed-drawn ensemble of vortex instances (batched workload): ranges,
-degrees, determinism and the per-degree stable CFL."""
+"""The seed-drawn ensembles of the batched workloads (vortex instances and
+the reference's Sod setup): ranges, degrees, determinism and the per-degree
+stable CFL."""
 import math
 
-from fvstuff_amd.capi import ensemble
+from fvstuff_amd.capi import ensemble, sod_ensemble
 
 
 def test_ensemble_ranges_and_determinism():
@@ -23,3 +24,17 @@ def test_ensemble_ranges_and_determinism():
     assert [i.vortex_eps for i in ics] == [i.vortex_eps for i in ics2]
     cfgs3, _, _ = ensemble(n=256, seed=7)
     assert [(c.K, c.p) for c in cfgs3] != [(c.K, c.p) for c in cfgs]
+
+
+def test_sod_ensemble_is_the_reference_setup():
+    cfgs, ics, cfls = sod_ensemble(n=128, seed=2024)
+    assert {c.p for c in cfgs} == {3, 5, 7}
+    for c, ic, cfl in zip(cfgs, ics, cfls):
+        # circular Sod, walls, Chandrashekar + HLLC, AV and limiter on
+        assert (c.volume_flux, c.surface_flux, c.bc, c.av, c.limiter) == (0, 0, 0, 1, 1)
+        assert (c.x0, c.y0, c.L) == (0.0, 0.0, 1.0) and c.precision == 1
+        assert ic.kind == 0 and 2.0 <= ic.rho_hi <= 11.0 and 2.0 <= ic.p_hi <= 10.0
+        assert 16 <= c.K <= 128
+        assert cfl == (0.5 if c.p == 3 else (0.3 if c.p == 5 else 0.2))
+    again = sod_ensemble(n=128, seed=2024)
+    assert [(c.K, c.p, i.rho_hi) for c, i in zip(again[0], again[1])] == [(c.K, c.p, i.rho_hi) for c, i in zip(cfgs, ics)]
