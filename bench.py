@@ -76,7 +76,7 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="vortex: skip the short samples of the other workloads (KH 1024^2 / 2048^2 p=3, "
-                         "256 batched instances)")
+                         "256 batched vortex instances, 256 batched Sod instances with AV + limiter)")
     ap.add_argument("--extra-steps", type=int, default=30,
                     help="timed steps of each short sample of the other workloads")
     ap.add_argument("--schedule", default="",
@@ -533,8 +533,9 @@ def main():
             log(f"[bench] vs oracle after {k} steps: {acc['vs_oracle']}")
 
     # short, bounded samples of the other workloads (after the headline's
-    # solvers are released): KH 1024^2 and 2048^2 p = 3 in both forms, and
-    # the 256-instance batch; rates over --extra-steps timed steps
+    # solvers are released): KH 1024^2 and 2048^2 p = 3 in both forms, the
+    # 256-instance vortex batch and the 256-instance batch of the reference's
+    # Sod setup (AV + limiter); rates over --extra-steps timed steps
     extra = None
     if not args.no_extra:
         del sols
@@ -544,6 +545,8 @@ def main():
                                                         tag=f"bench-kh{Kx}")
         for form in ("mixed", "fp64"):
             extra[f"batched_256_vortex_{form}"] = run_batched(256, 2024, form, args.extra_steps, 3)
+        for form in ("mixed", "fp64"):
+            extra[f"batched_256_sod_{form}"] = run_batched(256, 2024, form, args.extra_steps, 3, kind="sod")
 
     ref = hbm_reference(dof)
     ref["stage_kernels_over_reference"] = ref["ms_per_step"] / (res[main_form]["ms"] / args.steps)
