@@ -105,6 +105,20 @@ struct CTab {
     __device__ __forceinline__ CTab operator+(int o) const { return CTab{p + o}; }
 };
 
+// The same table read as plain global memory (vector loads after a pin or a
+// store).  The FP64 shock-capturing kernels at p <= 5 read the AV operators
+// (D for the velocity divergence, Lav) so: as scalar loads, next to the
+// volume matrix, their doubles overflow the scalar register file and the
+// compiler spills scalars into vector-register lanes (p = 3: 312 v_readlane
+// instead of 10; KH 2048^2 p = 3 FP64 1.4 % faster this way).  At p = 7 the
+// scalar loads are faster (1.8 %); FP32 compiles to scalar loads either way.
+template <typename Real>
+struct GTab {
+    const Real* p;
+    __device__ __forceinline__ Real operator[](int i) const { return p[i]; }
+    __device__ __forceinline__ GTab operator+(int o) const { return GTab{p + o}; }
+};
+
 // Host side: fill the derived constants of an argument block whose h, gamma,
 // K and P are set (the same IEEE divisions the device would do).
 inline void set_derived(KArgs& a) {
@@ -607,6 +621,8 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     const Gas64 gs{g, a.gm1, a.efloor};
 
     const CTab<Real> D{(const Real*)a.tab};
+    using AvTab = typename std::conditional<(sizeof(Real) == 8 && P <= 5), GTab<Real>, CTab<Real>>::type;
+    const AvTab Dav{(const Real*)a.tab};              // the AV's operators (GTab)
     const CTab<Real> winv = D + (N * N + N);
     const CTab<Real> Dv = D + (N * N + 2 * N);      // volume matrix
     // the tile's first node in every FP64 stream (variable q at + q*S)
@@ -892,7 +908,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             for (int i = 0; i < N; i++) {
                 Real d = 0;
 #pragma unroll
-                for (int k = 0; k < N; k++) d += D[i * N + k] * un[k];
+                for (int k = 0; k < N; k++) d += Dav[i * N + k] * un[k];
                 dst[le * N2 + lbase + i * lstride] = d;
             }
         }
@@ -1066,7 +1082,7 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         //    = -nu jac^2 Lav Uc, Lav = W^-1 D^T W D precomputed (one pass) ──
         if (FULL && !SD && !MFMA_AV && a.av) {
             const Real f = -sNu[le] * jac * jac;
-            const CTab<Real> Lav = D + (2 * N * N + 2 * N);
+            const AvTab Lav = Dav + (2 * N * N + 2 * N);
             Real uc[N][4];      // conserved variables of the line's nodes (u, v as rotated)
 #pragma unroll
             for (int k = 0; k < N; k++) {
