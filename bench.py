@@ -83,10 +83,11 @@ def parse():
                     help="E,X: elements per workgroup, XCD remap (implies --no-autotune)")
     ap.add_argument("--cpu-steps", type=int, default=15,
                     help="SSP-RK3 steps of the CPU oracle baseline (about 10 s on 16 host threads)")
-    ap.add_argument("--workload", default="vortex", choices=["vortex", "kh", "sd", "batched", "copy"],
+    ap.add_argument("--workload", default="vortex", choices=["vortex", "kh", "sd", "batched", "copy", "colloc"],
                     help="vortex: the headline line; kh: Kelvin-Helmholtz (--K, --p, --t-end); "
                          "sd: spectral-difference sibling on circular Sod (--K, --p, --steps); "
-                         "batched: seeded ensemble of vortex instances; copy: device copy probe")
+                         "batched: seeded ensemble of vortex instances; copy: device copy probe; "
+                         "colloc: the collocation volume term on the vortex (the MFMA A/B, --mfma)")
     ap.add_argument("--t-end", type=float, default=1.5)
     ap.add_argument("--no-shock-capturing", action="store_true",
                     help="kh: switch off the Ducros AV and the Zhang-Shu limiter")
@@ -95,7 +96,8 @@ def parse():
     ap.add_argument("--t-compare", type=float, default=0.01,
                     help="kh: early time of the mixed-vs-FP64 comparison (0 = none)")
     ap.add_argument("--mfma", type=int, default=-1, choices=[-1, 0, 1],
-                    help="kh: the AV contraction on the matrix cores (1), on the VALU (0), or tuned (-1)")
+                    help="kh: the AV contraction, colloc: the volume term, on the matrix cores (1), "
+                         "on the VALU (0), or tuned (-1)")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--persistent", type=int, default=-1, choices=[-1, 0, 1],
                     help="batched: persistent stage launches (1), one workgroup per tile (0), tuned (-1)")
@@ -431,6 +433,36 @@ def bench_batched(args):
                                  kind=args.ensemble)))
 
 
+def bench_colloc(args):
+    """The collocation volume term (dgsem_lobatto_2d.cu:431-439, EC_FLUX=0 of
+    the reference) on the vortex workload, Rusanov faces: the A/B of its
+    contraction on the VALU (--mfma 0) or on the matrix cores (--mfma 1, p = 3
+    mixed form; -1: the autotuner decides), timed with HIP events."""
+    import torch
+    from fvstuff_amd import dgsem as D
+    from fvstuff_amd.capi import vortex_config, make_ic
+    ic = make_ic("vortex")
+    K, p = args.K, args.p
+    dof = 4 * K * K * (p + 1) ** 2
+    sched = {}
+    if args.schedule:
+        E_, X_ = (int(v) for v in args.schedule.split(","))
+        sched = dict(elems_per_block=E_, xcd_remap=X_)
+    out = {}
+    for form in args.forms.split(","):
+        cfg = vortex_config(K=K, p=p, precision=form, volume_flux="collocation", mfma=args.mfma, **sched)
+        s, Q, ms, t, tuned = time_form(D, torch, cfg, ic, args.steps, args.warmup, args.cfl,
+                                       not (args.no_autotune or args.schedule))
+        rate = dof * 3 * args.steps / (ms * 1e-3)
+        out[form] = {"value": rate, "ms_per_step": ms / args.steps, "us_per_launch": ms * 1e3 / (3 * args.steps),
+                     "schedule": tuned, "hbm_frac": rate * (64.0 / 3.0) / 1e9 / HBM_PEAK_GBS, "t": t}
+        log(f"[bench-colloc] {form}: {out[form]}")
+        del Q, s
+    print(json.dumps({"workload": f"isentropic vortex {K}x{K} p={p}, collocation volume term + Rusanov, "
+                                  f"{args.steps} steps, CFL {args.cfl}, mfma {args.mfma}",
+                      "unit": "DOF*stage/s", "dof": dof, **out}))
+
+
 def bench_copy(args):
     """Device copy of one state vector (known bytes) for counter scaling."""
     import torch
@@ -458,6 +490,8 @@ def main():
         return bench_batched(args)
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "colloc":
+        return bench_colloc(args)
     import torch
     from fvstuff_amd import dgsem as D
     from fvstuff_amd.capi import vortex_config, make_ic

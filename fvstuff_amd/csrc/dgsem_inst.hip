@@ -92,6 +92,9 @@ static hipError_t launch_vol(const KArgs& a, int grid, int block, size_t lds, hi
 hipError_t DG_CAT(launch_stage_p, DG_P)(const KArgs& a, bool fp32, bool full, bool mfma, int grid, int block,
                                         size_t lds, hipStream_t s) {
 #if DG_P == 3
+    // the FP32 lean collocation kernel with its volume term on the matrix cores
+    if (mfma && fp32 && !full && a.vol == VOL_COLLOCATION)
+        return launch_one<float, false, VOL_COLLOCATION, true>(a, grid, block, lds, s);
     // the FP32 shock-capturing kernels with the AV on the matrix cores
     if (mfma && fp32 && full && a.vol != VOL_SD) {
         switch (a.vol) {

@@ -581,15 +581,17 @@ template <int P, int VOL, bool FULL> struct Occ<P, double, VOL, FULL> {
 // batched kernel, whose large workgroups are LDS-limited, runs faster so; the
 // single-solve kernels run slower, measured).
 // U0_EARLY: U^n is loaded with the stage input at the start (see below).
-// MFMA_AV (p = 3, FP32, shock-capturing kernels): the artificial viscosity's
-// tensor-product contraction on the matrix cores instead of in the line
-// phase (see the AV pre-pass below).
+// MFMA_AV (p = 3, FP32): the tile's linear tensor-product contraction on the
+// matrix cores instead of in the line phase: in the shock-capturing kernels
+// the artificial viscosity's (see the AV pre-pass below), in the lean
+// collocation kernel the volume term D f (see the collocation pre-pass).
 template <int P, typename Real, bool FULL, int VOL, bool PIN = (sizeof(Real) == 8), bool UP_LDS = true,
           bool U0_EARLY = !FULL, bool PAIR_OK = true, bool MFMA_AV = false>
 __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int tile, bool lead, int tid) {
     static_assert(UP_LDS || !FULL, "the limiter needs the stage values in LDS");
-    static_assert(!MFMA_AV || (P == 3 && sizeof(Real) == 4 && FULL && VOL != VOL_SD),
-                  "the MFMA form of the AV is the p = 3 FP32 shock-capturing kernel's");
+    static_assert(!MFMA_AV || (P == 3 && sizeof(Real) == 4 && VOL != VOL_SD && (FULL || VOL == VOL_COLLOCATION)),
+                  "the MFMA forms are the p = 3 FP32 shock-capturing and collocation kernels'");
+    constexpr bool MFMA_COL = MFMA_AV && !FULL && VOL == VOL_COLLOCATION;
     constexpr bool SD = VOL == VOL_SD;
     static_assert(UP_LDS || !SD, "the SD line reads the stage input from LDS");
     constexpr int N = P + 1, N2 = N * N;
@@ -639,6 +641,9 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     Real* sDv = sDu + E * N2;                       // [E][N2] AV: dv/dy
     Real* sNu = sDv + E * N2;                       // [E]     AV: element viscosity
     Real* sUc = sNu + E;                            // [E][4][N2] MFMA_AV: conserved nodes, then the AV
+    // [2][E][4][N2] MFMA collocation: the nodes' x- and y-fluxes; the x plane
+    // then holds the volume term D f
+    Real* sMf = (Real*)(smem + a.off_av);
     unsigned long long* sLamE = (unsigned long long*)(smem + a.off_lam);   // [E] max wave speed (bits)
     unsigned long long* sThE = sLamE + E;           // [E] max sensor (bits of the double)
     double* sEl = (double*)(smem + a.off_el);       // per-element scratch
@@ -859,12 +864,21 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         Real* p = sW + s * 5;
         p[0] = (Real)0.5 * w.r; p[1] = (Real)0.5 * w.u; p[2] = (Real)0.5 * w.v;
         p[3] = (Real)0.5 * w.p; p[4] = (Real)0.5 * w.z;
-        if (MFMA_AV && a.av) {
+        if (MFMA_AV && FULL && a.av) {
             // the conserved variables the AV acts on, from the prims as the
             // line phase's VALU form computes them
             Real* u = sUc + e * 4 * N2 + nd;
             u[0] = w.r; u[N2] = w.r * w.u; u[2 * N2] = w.r * w.v;
             u[3 * N2] = w.p * ph.inv_gm1 + (Real)0.5 * w.r * (w.u * w.u + w.v * w.v);
+        }
+        if (MFMA_COL) {
+            // the x- and y-fluxes the collocation pre-pass contracts
+            Real fx[4], gy[4];
+            ph.template flux<0>(w, fx);
+            ph.template flux<1>(w, gy);
+            Real* F = sMf + e * 4 * N2 + nd;
+#pragma unroll
+            for (int q = 0; q < 4; q++) { F[q * N2] = fx[q]; F[4 * E * N2 + q * N2] = gy[q]; }
         }
         if (PRIM32 && !SD && need_lam && staged) {
             // FP32 node speeds, reduced on their 32-bit patterns (they feed
@@ -895,6 +909,43 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
         }
     }
     __syncthreads();
+
+    // ── collocation pre-pass on the matrix cores: the volume term
+    //    sum_k D_ik f(U_k) along x plus sum_k D_jk g(U_k) along y (:431-439)
+    //    for every node, F_q D^T + D G_q on the 4 x 4 node arrays F_q, G_q of
+    //    each element and variable, with the block layout of the AV's MFMA
+    //    form below (block b = (element, variable), lane 4b + c; x-term step
+    //    m: A[r] = F_q[r][m], B[c] = D[c][m]; y-term: A[r] = D[r][m],
+    //    B[c] = G_q[m][c]); the result at node (jn = r, in = c) replaces F_q
+    //    and the line phase adds only the face terms.  Each wave reads and
+    //    writes the planes of the 8 elements of its own lines (reads first);
+    //    elements past the tile hold the stand-in state. ──
+    if (MFMA_COL) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const int l = tid & 63, c = l & 3, q = (l >> 2) & 3;
+        float dv[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) dv[m] = Dv[c * 4 + m];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int e = 8 * (tid >> 6) + 4 * h + (l >> 4);
+            const bool ok = e < E;
+            Real* Fq = sMf + (ok ? e : 0) * 4 * N2 + q * N2;
+            const Real* Gq = Fq + 4 * E * N2;
+            float fx[4], gy[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) { fx[m] = Fq[c * 4 + m]; gy[m] = Gq[m * 4 + c]; }
+            f4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int m = 0; m < 4; m++) d = __builtin_amdgcn_mfma_f32_4x4x1f32(fx[m], dv[m], d, 0, 0, 0);
+#pragma unroll
+            for (int m = 0; m < 4; m++) d = __builtin_amdgcn_mfma_f32_4x4x1f32(dv[m], gy[m], d, 0, 0, 0);
+            if (ok) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) Fq[r * 4 + c] = d[r];
+            }
+        }
+    }
 
     // ── AV pre-pass: velocity divergence, sensor, element viscosity (:483-521) ──
     if (FULL && !SD && a.av) {
@@ -1074,6 +1125,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
             const CTab<Real> Isf = D + (3 * N * N + 2 * N);     // [N+1][N]
             const CTab<Real> Dsd = Isf + (N + 1) * N;           // [N][N+1]
             sd_line<N, Real>(ph, sU + le * 4 * N2, lbase, lstride, ydir, Isf, Dsd, cm, cp, acc);
+        } else if (MFMA_COL) {
+            // the volume term came from the matrix cores: the face terms alone
+#pragma unroll
+            for (int i = 0; i < N; i++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[i][q] = i == 0 ? cm[q] : (i == N - 1 ? cp[q] : (Real)0);
         } else {
             line_volume<N, Real, VOL, PIN>(ph, hw, Dv, cm, cp, acc);
         }
@@ -1138,10 +1195,12 @@ __device__ __forceinline__ void stage_tile(const KArgs& a, const StepCtl sc, int
     __syncthreads();
 
     // ── C: node phase ──
-    // R of node nd of element e, variable q (with the MFMA form's AV)
+    // R of node nd of element e, variable q (with the matrix-core forms' AV or
+    // collocation volume term)
     auto rhs = [&](int e, int nd, int q) -> Real {
         const Real r = sR[e * 4 * N2 + q * N2 + nd];
-        if (MFMA_AV && a.av) return r + sUc[e * 4 * N2 + q * N2 + nd];
+        if (MFMA_AV && FULL && a.av) return r + sUc[e * 4 * N2 + q * N2 + nd];
+        if (MFMA_COL) return r + sMf[e * 4 * N2 + q * N2 + nd];
         return r;
     };
     if (a.mode == MODE_RHS) {
@@ -1403,7 +1462,9 @@ inline int align_up(int x, int a) { return (x + a - 1) / a * a; }
 
 // `stage`: the FP64 input state is staged in LDS (sU) for the update phase;
 // `lim`: the limiter's per-element scratch.
-// `mfma`: the MFMA form of the AV (its conserved-node planes, [E][4][N2]).
+// `mfma`: the matrix-core forms' planes: with av the AV's conserved-node
+// planes [E][4][N2] after its other arrays; without it the collocation
+// form's flux planes [2][E][4][N2].
 inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, bool lam, bool lim,
                             bool mfma = false) {
     const int N2 = N * N;
@@ -1413,6 +1474,7 @@ inline LdsLayout lds_layout(int N, int E, int real_bytes, bool stage, bool av, b
     if (stage) off += align_up(E * 4 * N2 * 8, 16);
     L.off_av = off;
     if (av) off += align_up((2 * E * N2 + E + (mfma ? 4 * E * N2 : 0)) * real_bytes, 16);
+    else if (mfma) off += align_up(8 * E * N2 * real_bytes, 16);
     L.off_lam = off;
     if (lam || av) off += align_up(2 * E * 8, 16);
     L.off_el = off;

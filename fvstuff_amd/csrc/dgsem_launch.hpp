@@ -5,8 +5,9 @@
 
 namespace dg {
 
-// Launches stage_kernel<P, Real, FULL> (mfma: its p = 3 FP32 shock-capturing
-// form with the AV on the matrix cores).  Returns the launch's hipError_t.
+// Launches stage_kernel<P, Real, FULL> (mfma: its p = 3 FP32 form with the AV
+// of the shock-capturing kernels, or the lean collocation kernel's volume
+// term, on the matrix cores).  Returns the launch's hipError_t.
 typedef hipError_t (*StageLauncher)(const KArgs& a, bool fp32, bool full, bool mfma, int grid, int block,
                                     size_t lds, hipStream_t s);
 // Limiter on E-element tiles in place.

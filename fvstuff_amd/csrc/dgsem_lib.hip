@@ -33,7 +33,7 @@ struct dg_ctx {
     double* d_xi = nullptr;              // LGL node coordinates (IC kernel)
     int part_blocks = 0;
     int E = 0, xcd = 0;
-    bool mfma = false;                   // AV contraction on the matrix cores (p = 3, FP32, av)
+    bool mfma = false;                   // contraction on the matrix cores (p = 3, FP32; AV or collocation)
     int last_steps = 0;                  // steps taken by the last dg_advance
     // graph replay of the fused loop: two steps per replay, captured on `cap`
     hipStream_t cap = nullptr;
@@ -558,8 +558,11 @@ static int default_E(int N, int NEL) {
 }
 
 // The MFMA form of the AV exists for the p = 3 FP32 shock-capturing kernels.
+// The matrix-core forms: the AV operator of the p = 3 mixed shock-capturing
+// kernels, and the volume term of the p = 3 mixed collocation kernel.
 static bool mfma_capable(const dg_config& f) {
-    return f.p == 3 && f.precision == DG_MIXED && f.av && f.scheme == DG_SCHEME_DGSEM;
+    return f.p == 3 && f.precision == DG_MIXED && f.scheme == DG_SCHEME_DGSEM &&
+           (f.av || f.volume_flux == DG_VOL_COLLOCATION);
 }
 
 // ── stage launch ───────────────────────────────────────────────────────────

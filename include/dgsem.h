@@ -95,10 +95,12 @@ typedef struct dg_config {
     int    elems_per_block; /* tuning: elements per workgroup, 0 = heuristic     */
     int    xcd_remap;       /* tuning: XCD-aware block->tile map, -1 = default on */
     int    scheme;          /* enum dg_scheme (DG_SCHEME_DGSEM by default)      */
-    int    mfma;            /* tuning: the artificial viscosity's tensor-product
-                               contraction on the matrix cores (p = 3, DG_MIXED,
-                               av on); -1 = dg_autotune decides (off until then),
-                               0 = off, 1 = on                                   */
+    int    mfma;            /* tuning: the tensor-product contraction on the
+                               matrix cores (p = 3, DG_MIXED): the artificial
+                               viscosity's (av on), or the collocation volume
+                               term's (DG_VOL_COLLOCATION, av off); -1 =
+                               dg_autotune decides (off until then), 0 = off,
+                               1 = on                                            */
 } dg_config;
 
 /* Initial-condition parameters (unused fields are ignored). */
