@@ -403,8 +403,10 @@ class Batch:
         return bool(g.value), l.value, bool(p.value), bool(x.value)
 
     def autotune(self, Qs, cfl):
-        """Time the workgroup sizes in both launch forms and keep the fastest;
-        returns the lanes per workgroup (see options() for the form)."""
+        """Time the workgroup sizes and orders in the launch forms (by degree
+        only for a batch with AV / limiter), then each degree's own size, and
+        keep the fastest; returns the common lanes per workgroup (see
+        options(), by_degree() and degree_lanes() for the schedule)."""
         cfl = [float(cfl)] * self.n if isinstance(cfl, (int, float)) else [float(c) for c in cfl]
         g = C.c_int()
         self._check(self.lib.dg_batch_autotune(self._b, self._ptrs(Qs), (C.c_double * self.n)(*cfl), C.byref(g)))

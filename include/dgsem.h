@@ -341,8 +341,9 @@ int         dg_batch_get_degree_lanes(const dg_batch* b, int p, int* lanes);
 /* Current options (any pointer may be NULL). */
 int         dg_batch_get_options(const dg_batch* b, int* use_graph, int* lanes, int* persistent, int* xcd_bins);
 /* Time the three workgroup sizes, each with both orders in the three launch
- * forms (all-degree kernel one workgroup per tile, persistent, by degree), on
- * the given states (they are advanced by a few steps) and keep the fastest;
+ * forms (all-degree kernel one workgroup per tile, persistent, by degree; a
+ * batch with AV / limiter: the by-degree form only), on the given states
+ * (they are advanced by a few steps) and keep the fastest;
  * in the by-degree form then refine each degree's workgroup size on its own
  * (kept if the step runs >= 1 % faster).  Writes the common size to *lanes if
  * non-NULL. */
