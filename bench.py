@@ -182,7 +182,7 @@ def cpu_model():
 
 def cpu_share():
     """Host threads this process may use: the OpenMP thread count the
-    environment sets (the GPU box gives each GPU a share of its cores and
+    environment sets (a shared host that gives each GPU a share of its cores
     sets it), else the CPUs in the process's affinity mask."""
     try:
         aff = len(os.sched_getaffinity(0))
