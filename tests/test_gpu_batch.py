@@ -300,7 +300,8 @@ def test_batch_shock_capturing_matches_single_solves(D, precision):
     """The reference's default configuration (circular Sod, walls, AV,
     limiter) batched over degrees 1..7: every instance gets the bits of its
     own dg_advance at every workgroup size, in both block orders, with and
-    without graph replay."""
+    without graph replay, with per-degree workgroup sizes and after the
+    autotuner (which keeps the by-degree form)."""
     cfgs, ics = _sod_ensemble(precision)
     cfl = 0.3
     ref = []
@@ -322,6 +323,23 @@ def test_batch_shock_capturing_matches_single_solves(D, precision):
                     key = (lanes, bins, use_graph, cfgs[i].K, cfgs[i].p)
                     assert ts[i] == t, key
                     assert np.array_equal(got[i], Q), key
+    # per-degree workgroup sizes, then the autotuner's schedule
+    for step in ("degree_lanes", "autotune"):
+        if step == "degree_lanes":
+            b.set_options(use_graph=True, lanes=64)
+            for p, lanes in ((7, 128), (5, 256), (3, 64), (1, 128)):
+                b.set_degree_lanes(p, lanes)
+        else:
+            Qs = b.init_states(ics)
+            b.autotune(Qs, [cfl] * len(cfgs))
+            assert b.by_degree()
+        Qs = b.init_states(ics)
+        ts = b.advance(Qs, 8, [cfl] * len(cfgs))
+        got = host(Qs)
+        for i, (t, Q) in enumerate(ref):
+            key = (step, b.options(), cfgs[i].K, cfgs[i].p, b.degree_lanes(cfgs[i].p))
+            assert ts[i] == t, key
+            assert np.array_equal(got[i], Q), key
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("mixed", 1e-6)])
