@@ -416,7 +416,10 @@ def run_batched(instances, seed, form, steps, warmup, lanes=0, persistent=-1, xc
         fn(Qs, steps, cfls)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        res[mode] = {"value": b.dofs * 3 * steps / wall, "ms_per_step": wall * 1e3 / steps}
+        rate = b.dofs * 3 * steps / wall
+        # algorithmic HBM share: 64/3 bytes per DOF*stage (the state streams)
+        res[mode] = {"value": rate, "ms_per_step": wall * 1e3 / steps,
+                     "hbm_frac": rate * (64.0 / 3.0) / 1e9 / HBM_PEAK_GBS}
         log(f"[bench-batched] {form} {mode}: {res[mode]}")
     what = "circular-Sod instances with AV and limiter" if kind == "sod" else "vortex instances"
     return {"workload": f"{instances} {what} (seed {seed}), {form}", "unit": "DOF*stage/s",
