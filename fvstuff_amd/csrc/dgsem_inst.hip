@@ -171,6 +171,12 @@ static hipError_t launch_degree_vol(const KArgs* args, const int2* tiles, int nt
             return launch_degree_one<Real, FULL, VOL_RANOCHA>(args, tiles, ntiles, stage, sc, block, lds, xcd, s);
         case VOL_CHANDRASHEKAR:
             return launch_degree_one<Real, FULL, VOL_CHANDRASHEKAR>(args, tiles, ntiles, stage, sc, block, lds, xcd, s);
+        case VOL_SD:
+            // the spectral-difference sibling has a lean kernel only
+            if constexpr (!FULL)
+                return launch_degree_one<Real, false, VOL_SD>(args, tiles, ntiles, stage, sc, block, lds, xcd, s);
+            else
+                return hipErrorInvalidValue;
         default:
             return launch_degree_one<Real, FULL, VOL_COLLOCATION>(args, tiles, ntiles, stage, sc, block, lds, xcd, s);
     }

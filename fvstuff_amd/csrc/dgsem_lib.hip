@@ -1269,6 +1269,9 @@ struct dg_batch {
     // AV and / or the limiter on (every instance alike): the shock-capturing
     // stage kernels, which run in the by-degree form only
     bool full = false;
+    // spectral-difference instances (every instance alike): the SD stage
+    // kernels, in the by-degree form only
+    bool sd = false;
     int deg_lanes[PMAX_GPU + 1] = {0};  // by-degree form: lanes per workgroup of degree p (0 = lanes)
     hipGraphExec_t gexec = nullptr;
     char err[512] = {0};
@@ -1295,7 +1298,8 @@ static hipError_t batch_launch_stage(dg_batch* b, int stage, StepCtl sc) {
     if (b->by_degree) {
         for (const dg_batch::DegreeRun& r : b->runs) {
             hipError_t e = batch_degree_launcher(r.p)(b->d_args, b->d_tiles + r.first, r.count, stage, sc, fp32,
-                                                      b->full, b->common.volume_flux, r.block, r.lds,
+                                                      b->full, b->sd ? (int)VOL_SD : b->common.volume_flux,
+                                                      r.block, r.lds,
                                                       b->xcd_bins ? 1 : 0, b->cap);
             if (e != hipSuccess) return e;
         }
@@ -1323,6 +1327,10 @@ static int batch_lanes_of(const dg_batch* b, int p) {
 
 // whether the batched kernels keep the stage input in LDS: the all-degree
 // kernel by precision, the by-degree form always (the single-solve schedule)
+// Batches that only the by-degree form runs: shock capturing, or the SD
+// scheme (the all-degree kernel carries neither).
+static bool batch_degree_only(const dg_batch* b) { return b->full || b->sd; }
+
 static bool batch_up_lds(const dg_batch* b) {
     if (b->by_degree) return true;
     return b->common.precision == DG_MIXED ? BatchOcc<float>::up_lds : BatchOcc<double>::up_lds;
@@ -1335,7 +1343,8 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
         const dg_config& f = c->cfg;
         const bool fp32 = f.precision == DG_MIXED;
         const int E = batch_elems_per_block(c->N, batch_lanes_of(b, c->cfg.p));
-        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(b), f.av != 0, false, f.limiter != 0);
+        // (the SD kernel's nodal repair needs no limiter scratch)
+        LdsLayout L = lds_layout(c->N, E, fp32 ? 4 : 8, batch_up_lds(b), f.av != 0, false, !b->sd && f.limiter != 0);
         for (int s = 1; s <= 3; s++) {
             KArgs a;
             memset(&a, 0, sizeof(a));
@@ -1347,7 +1356,8 @@ static void batch_bind(dg_batch* b, double* const* Qs, const double* cfl, double
             a.w64 = (const double*)c->d_tab64 + c->N2;
             a.st = c->d_st;
             a.P = f.p; a.K = f.K; a.NEL = c->NEL; a.E = E; a.xcd = 0;
-            a.mode = MODE_STAGE; a.stage = s; a.vol = f.volume_flux; a.surf = f.surface_flux; a.bc = f.bc;
+            a.mode = MODE_STAGE; a.stage = s; a.vol = b->sd ? (int)VOL_SD : f.volume_flux;
+            a.surf = f.surface_flux; a.bc = f.bc;
             a.av = f.av; a.limiter = f.limiter;
             a.h = c->h; a.gamma = f.gamma;
             a.ca = s == 1 ? 0.0 : (s == 2 ? 0.75 : 1.0 / 3.0);
@@ -1406,7 +1416,7 @@ static hipError_t batch_tiles(dg_batch* b) {
                 const int E = batch_elems_per_block(c->N, batch_lanes_of(b, c->cfg.p));
                 r.block = std::min(BLOCK, (E * 2 * c->N + 63) / 64 * 64);
                 r.lds = (size_t)lds_layout(c->N, E, fp32 ? 4 : 8, true, c->cfg.av != 0, false,
-                                           c->cfg.limiter != 0).total;
+                                           !b->sd && c->cfg.limiter != 0).total;
             }
             r.count = (int)tiles.size() - r.first;
             if (r.count > 0) b->runs.push_back(r);
@@ -1500,13 +1510,14 @@ int dg_batch_create(const dg_config* cfgs, int n, void* stream, dg_batch** out) 
         const dg_config& f = cfgs[i];
         if (f.volume_flux != c0.volume_flux || f.surface_flux != c0.surface_flux || f.bc != c0.bc ||
             f.precision != c0.precision || (f.av != 0) != (c0.av != 0) || (f.limiter != 0) != (c0.limiter != 0) ||
-            f.scheme != DG_SCHEME_DGSEM)
+            f.scheme != c0.scheme)
             return DG_ERR_INVALID_ARG;
     }
     dg_batch* b = new dg_batch();
     b->n = n;
-    b->full = c0.av || c0.limiter;
-    b->by_degree = b->full;
+    b->sd = c0.scheme == DG_SCHEME_SD;
+    b->full = !b->sd && (c0.av || c0.limiter);
+    b->by_degree = batch_degree_only(b);
     b->stream = (hipStream_t)stream;
     b->common = c0;
     b->ctx.assign(n, nullptr);
@@ -1577,8 +1588,9 @@ int dg_batch_set_xcd_bins(dg_batch* b, int xcd_bins);
 
 int dg_batch_set_persistent(dg_batch* b, int persistent) {
     if (!b) return DG_ERR_INVALID_ARG;
-    if (persistent && b->full)
-        return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_set_persistent: AV / limiter run in the by-degree form only");
+    if (persistent && batch_degree_only(b))
+        return bfail(b, DG_ERR_INVALID_ARG,
+                     "dg_batch_set_persistent: AV / limiter and the SD scheme run in the by-degree form only");
     if (persistent && b->by_degree) {   // the persistent form is the all-degree kernel's
         b->by_degree = false;
         BHIPCK(b, batch_tiles(b));
@@ -1677,7 +1689,7 @@ int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* la
     // kernel with one workgroup per tile or persistent, one launch per degree)
     for (int cand = 0; cand < 18 && !rc; cand++) {
         const int l = cands[cand / 6], form = (cand % 6) >> 1, bins = cand & 1;
-        if (b->full && form != 2) continue;   // the shock-capturing kernels: by degree only
+        if (batch_degree_only(b) && form != 2) continue;   // shock capturing, SD: by degree only
         if ((rc = dg_batch_set_options(b, b->use_graph, l))) break;
         if ((rc = dg_batch_set_by_degree(b, form == 2))) break;
         if ((rc = dg_batch_set_xcd_bins(b, bins))) break;
@@ -1730,8 +1742,9 @@ int dg_batch_get_options(const dg_batch* b, int* use_graph, int* lanes, int* per
 
 int dg_batch_set_by_degree(dg_batch* b, int by_degree) {
     if (!b) return DG_ERR_INVALID_ARG;
-    if (!by_degree && b->full)
-        return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_set_by_degree: AV / limiter run in the by-degree form only");
+    if (!by_degree && batch_degree_only(b))
+        return bfail(b, DG_ERR_INVALID_ARG,
+                     "dg_batch_set_by_degree: AV / limiter and the SD scheme run in the by-degree form only");
     if ((by_degree != 0) == b->by_degree) return DG_OK;
     b->by_degree = by_degree != 0;
     if (b->by_degree) b->persistent = false;

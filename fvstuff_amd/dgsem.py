@@ -300,7 +300,8 @@ class Batch:
     """Many independent solves (own K, p, domain, IC, CFL) stepped together:
     one launch per stage for all instances (one workgroup per tile), or one per
     degree present (set_by_degree), graph-replayed.  AV / limiter must be on
-    in all instances or in none; with them the batch runs by degree only."""
+    in all instances or in none, and the scheme alike; with them, or with the
+    spectral-difference scheme, the batch runs by degree only."""
 
     def __init__(self, cfgs, device="cuda:0", stream=None):
         if not torch.cuda.is_available():
@@ -315,7 +316,7 @@ class Batch:
         self._b = C.c_void_p()
         rc = self.lib.dg_batch_create(arr, self.n, C.c_void_p(self.stream.cuda_stream), C.byref(self._b))
         if rc != 0:
-            raise DGError(rc, "dg_batch_create failed (flux / boundary / precision / AV / limiter switches differ between instances, a non-DGSEM scheme, or allocation)")
+            raise DGError(rc, "dg_batch_create failed (flux / boundary / precision / AV / limiter / scheme switches differ between instances, or allocation)")
         self.sizes = [int(self.lib.dg_batch_state_size(self._b, i)) for i in range(self.n)]
         self.dofs = sum(self.sizes)
 

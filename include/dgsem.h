@@ -63,7 +63,8 @@ enum dg_precision {
  * nodes (compute_sd_rhs, fr_lobatto_2d.cu:332-525).  It has no volume flux
  * and no artificial viscosity (av must be 0); its `limiter` is the
  * reference's nodal positivity repair (fr_lobatto_2d.cu:589-615) instead of
- * the Zhang-Shu limiter.  Not available in batched mode. */
+ * the Zhang-Shu limiter.  Batched SD solves (every instance SD) run in the
+ * by-degree form only. */
 enum dg_scheme {
     DG_SCHEME_DGSEM = 0,       /* dgsem_lobatto_2d.cu                            */
     DG_SCHEME_SD    = 1        /* fr_lobatto_2d.cu                               */
@@ -289,11 +290,13 @@ int         dg_load_state(dg_ctx* ctx, double* Q, double* t, long long* steps, c
  * degree, each with the single-solve kernel of that degree); each instance
  * keeps its own device-side dt and time.  The stage launches of the time loop
  * are replayed from a captured HIP graph.  AV and the limiter must be on in
- * every instance or in none (DG_ERR_INVALID_ARG otherwise); such a batch runs
- * in the by-degree form only, with the single-solve shock-capturing kernel of
- * each degree, and refuses dg_batch_set_by_degree(b, 0) and
- * dg_batch_set_persistent(b, 1).  Their parameters (av_cav, av_ksensor,
- * eps_rho, eps_p) may differ between instances. */
+ * every instance or in none, and the scheme alike (DG_ERR_INVALID_ARG
+ * otherwise); a batch with AV / limiter, or of spectral-difference
+ * instances, runs in the by-degree form only, with the single-solve kernel of
+ * each degree (shock-capturing, or SD with its nodal repair), and refuses
+ * dg_batch_set_by_degree(b, 0) and dg_batch_set_persistent(b, 1).  The AV /
+ * limiter parameters (av_cav, av_ksensor, eps_rho, eps_p) may differ between
+ * instances. */
 typedef struct dg_batch dg_batch;
 
 int         dg_batch_create(const dg_config* cfgs, int n, void* stream, dg_batch** out);

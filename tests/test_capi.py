@@ -85,8 +85,8 @@ def test_struct_layout_matches_header(tmp_path):
 def test_scheme_validation_before_any_device_call():
     """dg_create / dg_batch_create reject invalid configurations (empty grid,
     degree 0, bad domain or gas, AV or an unknown scheme with spectral
-    difference, batched spectral difference) during validation, before any
-    HIP call."""
+    difference, a batch that mixes spectral difference and DGSEM) during
+    validation, before any HIP call."""
     lib = load_library()
     ctx = C.c_void_p()
     for bad in (capi.sd_config(K=4, p=3, av=True), capi.make_config(K=4, p=3, scheme=2),
@@ -95,9 +95,9 @@ def test_scheme_validation_before_any_device_call():
                 capi.make_config(K=4, p=3, mfma=2), capi.make_config(K=4, p=3, mfma=-2)):
         assert lib.dg_create(C.byref(bad), None, C.byref(ctx)) == capi.DG_ERR_INVALID_ARG
         assert not ctx.value
-    arr = (capi.DGConfig * 1)(capi.sd_config(K=4, p=3, limiter=False))
+    arr = (capi.DGConfig * 2)(capi.sd_config(K=4, p=3, limiter=False), capi.make_config(K=4, p=3, limiter=False))
     b = C.c_void_p()
-    assert lib.dg_batch_create(arr, 1, None, C.byref(b)) == capi.DG_ERR_INVALID_ARG
+    assert lib.dg_batch_create(arr, 2, None, C.byref(b)) == capi.DG_ERR_INVALID_ARG
     assert lib.dg_sd_operators(None, None, None, None) == capi.DG_ERR_INVALID_ARG
 
 

@@ -373,3 +373,87 @@ def test_batch_shock_capturing_matches_oracle(D, precision, tol):
         print(f"instance {i} K={cfg.K} p={cfg.p} p_hi={ic.p_hi:g}: rel L2 vs oracle {err:.2e}")
         assert err < tol, (i, cfg.K, cfg.p, err)
         assert ts[i] == pytest.approx(t, rel=1e-12 if precision == "fp64" else 1e-6)
+
+
+def _sd_ensemble(precision, kind):
+    """Spectral-difference instances (fr_lobatto_2d.cu), every switch alike
+    within a batch: "sod", the reference driver's circular Sod with walls,
+    HLLC and the nodal repair at p = 1..4; "vortex", periodic vortices with
+    Rusanov faces at p = 1..7."""
+    from fvstuff_amd.capi import sd_config
+    cfgs, ics, cfls = [], [], []
+    if kind == "sod":
+        for p, K in ((1, 12), (2, 10), (3, 9), (4, 7)):
+            cfgs.append(sd_config(K=K, p=p, precision=precision))
+            ics.append(make_ic("sod"))
+            cfls.append(0.1 / p)
+    else:
+        for p, K in ((1, 12), (2, 9), (3, 8), (4, 6), (5, 5), (6, 4), (7, 4)):
+            cfgs.append(sd_config(K=K, p=p, precision=precision, L=10.0, bc="periodic", surface_flux="rusanov",
+                                  limiter=False))
+            ics.append(make_ic("vortex"))
+            cfls.append(0.2 if p <= 5 else 0.15)
+    return cfgs, ics, cfls
+
+
+@pytest.mark.parametrize("kind", ["sod", "vortex"])
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_batch_sd_matches_single_solves(D, precision, kind):
+    """Batched spectral-difference solves (by-degree form only) give every
+    instance the bits of its own dg_advance at every workgroup size, in both
+    block orders, with and without graph replay; the all-degree and
+    persistent forms are refused; a batch mixing SD and DGSEM is refused."""
+    cfgs, ics, cfls = _sd_ensemble(precision, kind)
+    ref = []
+    for cfg, ic, cfl in zip(cfgs, ics, cfls):
+        s = D.Solver(cfg)
+        Q = s.init_state(ic)
+        t, _ = s.advance(Q, 8, cfl)
+        ref.append((t, host([Q])[0]))
+    b = D.Batch(cfgs)
+    assert b.by_degree()
+    with pytest.raises(D.DGError, match="by-degree form only"):
+        b.set_by_degree(False)
+    with pytest.raises(D.DGError, match="by-degree form only"):
+        b.set_persistent(True)
+    for lanes in (64, 256):
+        for bins in (False, True):
+            for use_graph in (True, False):
+                b.set_options(use_graph=use_graph, lanes=lanes)
+                b.set_xcd_bins(bins)
+                Qs = b.init_states(ics)
+                ts = b.advance(Qs, 8, cfls)
+                got = host(Qs)
+                for i, (t, Q) in enumerate(ref):
+                    key = (lanes, bins, use_graph, cfgs[i].K, cfgs[i].p)
+                    assert ts[i] == t, key
+                    assert np.array_equal(got[i], Q), key
+    with pytest.raises(D.DGError):
+        D.Batch([cfgs[0], make_config(K=6, p=3, precision=precision)])
+
+
+@pytest.mark.parametrize("kind", ["sod", "vortex"])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-6)])
+def test_batch_sd_matches_oracle(D, precision, tol, kind):
+    """The SD ensemble after 20 batched steps against the FP64 oracle run of
+    each instance: FP64 within 1e-12 relative L2, mixed within 1e-6 (the
+    bars of the single-solve SD loops)."""
+    from oracle.oracle import Oracle, lgl
+    from tests.layout import gll_weights_2d, rel_l2
+    cfgs, ics, cfls = _sd_ensemble(precision, kind)
+    b = D.Batch(cfgs)
+    Qs = b.init_states(ics)
+    ts = b.advance(Qs, 20, cfls)
+    got = host(Qs)
+    for i, (cfg, ic, cfl) in enumerate(zip(cfgs, ics, cfls)):
+        c64 = type(cfg).from_buffer_copy(cfg)
+        c64.precision = 0
+        o = Oracle(c64)
+        Q = o.init_state(ic)
+        rc, t, _ = o.advance(Q, 20, cfl)
+        assert rc == 0
+        _, w = lgl(cfg.p)
+        err = rel_l2(got[i], Q, gll_weights_2d(w, cfg.K, cfg.L / cfg.K))
+        print(f"SD instance {i} K={cfg.K} p={cfg.p}: rel L2 vs oracle {err:.2e}")
+        assert err < tol, (i, cfg.K, cfg.p, err)
+        assert ts[i] == pytest.approx(t, rel=1e-12 if precision == "fp64" else 1e-6)
