@@ -1561,6 +1561,22 @@ long long dg_batch_state_size(const dg_batch* b, int i) {
     return (b && i >= 0 && i < b->n) ? dg_state_size(b->ctx[i]) : 0;
 }
 
+int dg_batch_save_state(dg_batch* b, int i, const double* Q, double t, long long steps, const char* path) {
+    if (!b) return DG_ERR_INVALID_ARG;
+    if (i < 0 || i >= b->n) return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_save_state: instance %d of %d", i, b->n);
+    const int rc = dg_save_state(b->ctx[i], Q, t, steps, path);
+    if (rc != DG_OK) bfail(b, rc, "instance %d: %s", i, dg_last_error(b->ctx[i]));
+    return rc;
+}
+
+int dg_batch_load_state(dg_batch* b, int i, double* Q, double* t, long long* steps, const char* path) {
+    if (!b) return DG_ERR_INVALID_ARG;
+    if (i < 0 || i >= b->n) return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_load_state: instance %d of %d", i, b->n);
+    const int rc = dg_load_state(b->ctx[i], Q, t, steps, path);
+    if (rc != DG_OK) bfail(b, rc, "instance %d: %s", i, dg_last_error(b->ctx[i]));
+    return rc;
+}
+
 int dg_batch_init_state(dg_batch* b, const dg_ic_params* ics, double* const* Qs) {
     if (!b || !ics || !Qs) return bfail(b, DG_ERR_INVALID_ARG, "dg_batch_init_state: null argument");
     for (int i = 0; i < b->n; i++) {

@@ -50,6 +50,8 @@ SIGNATURES = {
     "dg_sensor": (C.c_int, [_vp, _vp, _vp]),
     "dg_totals": (C.c_int, [_vp, _vp, _dp]),
     "dg_save_state": (C.c_int, [_vp, _vp, C.c_double, C.c_longlong, C.c_char_p]),
+    "dg_batch_save_state": (C.c_int, [_vp, C.c_int, _vp, C.c_double, C.c_longlong, C.c_char_p]),
+    "dg_batch_load_state": (C.c_int, [_vp, C.c_int, _vp, _dp, C.POINTER(C.c_longlong), C.c_char_p]),
     "dg_load_state": (C.c_int, [_vp, _vp, _dp, C.POINTER(C.c_longlong), C.c_char_p]),
     "dg_totals_noalloc": (C.c_int, [C.POINTER(DGConfig), _vp, _vp, _vp]),
     "dg_vortex_exact": (C.c_int, [_vp, C.POINTER(DGICParams), C.c_double, C.c_int, _vp]),
@@ -366,6 +368,16 @@ class Batch:
     def advance_each(self, Qs, nsteps, cfl, t_end=0.0, t=None):
         """The same steps, one dg_advance per instance (for comparison)."""
         return self._advance(self.lib.dg_batch_advance_each, Qs, nsteps, cfl, t_end, t)
+
+    def save_state(self, i, path, Q, t, steps):
+        """Checkpoint of instance i: its state Q (device), time and step count."""
+        self._check(self.lib.dg_batch_save_state(self._b, int(i), _ptr(Q), float(t), int(steps), os.fsencode(path)))
+
+    def load_state(self, i, path, Q):
+        """Restart of instance i from its checkpoint into Q; returns (t, steps)."""
+        t, n = C.c_double(), C.c_longlong()
+        self._check(self.lib.dg_batch_load_state(self._b, int(i), _ptr(Q), C.byref(t), C.byref(n), os.fsencode(path)))
+        return t.value, n.value
 
     def set_options(self, use_graph=True, lanes=0):
         self._check(self.lib.dg_batch_set_options(self._b, int(bool(use_graph)), int(lanes)))

@@ -306,6 +306,13 @@ const char* dg_batch_last_error(const dg_batch* b);
 long long   dg_batch_state_size(const dg_batch* b, int i);
 /* Qs[i]: device state of instance i (caller-owned), ics[i]: its IC */
 int         dg_batch_init_state(dg_batch* b, const dg_ic_params* ics, double* const* Qs);
+/* Checkpoint / restart of instance i: dg_save_state / dg_load_state with the
+ * instance's discretisation (the same file format); a batch resumed from its
+ * instances' checkpoints (their times passed back through t_inout) takes the
+ * same steps, bit for bit, as one that continues in a new dg_batch_advance
+ * call. */
+int         dg_batch_save_state(dg_batch* b, int i, const double* Q, double t, long long steps, const char* path);
+int         dg_batch_load_state(dg_batch* b, int i, double* Q, double* t, long long* steps, const char* path);
 /* nsteps fused SSP-RK3 steps of every instance; cfl[i] per instance; t_inout[i]
  * in/out (may be NULL); status_out[i] per-instance DG_* code (may be NULL).
  * Returns the first instance failure. */

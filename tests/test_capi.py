@@ -59,6 +59,8 @@ def test_host_only_calls():
     # checkpoint calls reject null arguments before touching a device
     assert lib.dg_save_state(None, None, 0.0, 0, b"x") == capi.DG_ERR_INVALID_ARG
     assert lib.dg_load_state(None, None, None, None, b"x") == capi.DG_ERR_INVALID_ARG
+    assert lib.dg_batch_save_state(None, 0, None, 0.0, 0, b"x") == capi.DG_ERR_INVALID_ARG
+    assert lib.dg_batch_load_state(None, 0, None, None, None, b"x") == capi.DG_ERR_INVALID_ARG
 
 
 def test_struct_layout_matches_header(tmp_path):
