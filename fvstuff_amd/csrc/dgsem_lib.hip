@@ -1194,32 +1194,41 @@ int dg_autotune(dg_ctx* c, int* elems_per_block, int* xcd_remap) {
     // the two forms of the AV contraction (VALU in the line phase, MFMA) when
     // the configuration leaves the choice to the tuner
     const int nm = (c->cfg.mfma < 0 && mfma_capable(c->cfg)) ? 2 : 1;
-    // every candidate starts from the same smooth state and takes a few small
-    // steps (CFL 0.1 is stable for every degree), so numerics never interfere;
-    // a candidate's time is the better of two 4-step runs (schedules within
-    // 1 % of each other are otherwise ranked by noise)
-    for (int mi = 0; mi < nm && rc == DG_OK; mi++)
-    for (int x = 0; x < 2 && rc == DG_OK; x++)
-        for (int E : Es) {
-            if (nm == 2) c->mfma = mi != 0;
+    // Every candidate runs from the same smooth state with a few small steps
+    // (CFL 0.1 is stable for every degree), so numerics never interfere.  The
+    // candidates are timed round-robin, four rounds of 8-step runs (24
+    // launches each), and a candidate's time is its best round: schedules
+    // 0.7-2 % apart (vortex 512^2 p = 7: 8 elements per workgroup against 16
+    // or 4) are otherwise ranked by noise and clock drift, and one candidate
+    // after another would see the drift as a difference.
+    struct Cand { int m, x, E; float t; };
+    std::vector<Cand> cands;
+    for (int mi = 0; mi < nm; mi++)
+        for (int x = 0; x < 2; x++)
+            for (int E : Es) cands.push_back({mi, x, E, 1e30f});
+    for (int round = -1; round < 4 && rc == DG_OK; round++)
+        for (Cand& cd : cands) {
+            if (nm == 2) c->mfma = cd.m != 0;
             double t = 0.0;
             if ((rc = dg_init_state(c, &ic, Q))) break;
-            rc = advance_impl(c, Q, 1, 0.1, 0.0, &t, nullptr, E, x, false);   // warm-up (LDS attribute, caches)
-            if (rc) break;
-            float cand = 1e30f;
-            for (int rep = 0; rep < 2 && rc == DG_OK; rep++) {
-                hipEventRecord(e0, c->stream);
-                rc = advance_impl(c, Q, 4, 0.1, 0.0, &t, nullptr, E, x, false);
-                hipEventRecord(e1, c->stream);
-                hipEventSynchronize(e1);
-                float ms = 0.f;
-                hipEventElapsedTime(&ms, e0, e1);
-                cand = std::min(cand, ms);
+            if (round < 0) {   // warm-up (LDS attribute, caches)
+                if ((rc = advance_impl(c, Q, 1, 0.1, 0.0, &t, nullptr, cd.E, cd.x, false))) break;
+                continue;
             }
+            hipEventRecord(e0, c->stream);
+            rc = advance_impl(c, Q, 8, 0.1, 0.0, &t, nullptr, cd.E, cd.x, false);
+            hipEventRecord(e1, c->stream);
+            hipEventSynchronize(e1);
             if (rc) break;
-            if (cand < best) { best = cand; bestE = E; bestX = x; bestM = c->mfma; }
-            if (x == 1 && cand < bestXcd) { bestXcd = cand; xcdE = E; xcdM = c->mfma; }
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            cd.t = std::min(cd.t, ms);
         }
+    for (const Cand& cd : cands) {
+        const bool m = nm == 2 ? cd.m != 0 : c->mfma;
+        if (cd.t < best) { best = cd.t; bestE = cd.E; bestX = cd.x; bestM = m; }
+        if (cd.x == 1 && cd.t < bestXcd) { bestXcd = cd.t; xcdE = cd.E; xcdM = m; }
+    }
     // The XCD order reads fewer bytes (a tile's face neighbours in its own
     // XCD's L2: vortex 512^2 p = 7, 1.004x the algorithmic traffic with it,
     // 1.05-1.19x without), so it is kept when it runs within 1 % of the

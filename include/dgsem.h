@@ -263,9 +263,12 @@ int         dg_error_norms(dg_ctx* ctx, const double* Q, const double* Qref, dou
  * (row-major, j = 0 at the bottom).  FP64, deterministic. */
 int         dg_frame_fields(dg_ctx* ctx, const double* Q, double* rho_el, double* p_el);
 
-/* Tuning: time a few candidate schedules (elements per block x XCD remap)
- * on scratch buffers and keep the fastest for this ctx.  Optional; dg_create
- * picks a heuristic schedule.  Writes the chosen values if pointers non-NULL. */
+/* Tuning: time the candidate schedules (elements per block x XCD remap, and
+ * the matrix-core form where the configuration leaves it open) on scratch
+ * buffers, round-robin over four rounds of 8-step runs, and keep the fastest
+ * for this ctx (the XCD order when it runs within 1 % of the fastest: it
+ * reads fewer bytes).  Optional; dg_create picks a heuristic schedule.
+ * Writes the chosen values if pointers non-NULL. */
 int         dg_autotune(dg_ctx* ctx, int* elems_per_block, int* xcd_remap);
 
 /* ── Checkpoint / restart (no counterpart in the reference, whose frames are
