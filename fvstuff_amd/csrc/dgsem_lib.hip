@@ -1691,41 +1691,53 @@ int dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* la
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    // one step (warm-up, graph instantiation), then 3 timed steps; the
-    // states are advanced by 4 steps per candidate at cfl[i]
+    // Every option change drops the cached graph: 3 untimed steps rebuild it
+    // (one eager step, instantiation, one replay) and warm the caches, then 6
+    // steps are timed (3 replays with the graph on); the states are advanced
+    // by 9 steps per timing at cfl[i]
     auto time_steps = [&](float* ms) -> int {
-        int r = dg_batch_advance(b, Qs, 1, cfl, 0.0, nullptr, nullptr);
+        int r = dg_batch_advance(b, Qs, 3, cfl, 0.0, nullptr, nullptr);
         if (r) return r;
         hipEventRecord(e0, b->stream);
-        if ((r = dg_batch_advance(b, Qs, 3, cfl, 0.0, nullptr, nullptr))) return r;
+        if ((r = dg_batch_advance(b, Qs, 6, cfl, 0.0, nullptr, nullptr))) return r;
         hipEventRecord(e1, b->stream);
         hipEventSynchronize(e1);
         *ms = 0.f;
         hipEventElapsedTime(ms, e0, e1);
         return DG_OK;
     };
-    float best = 1e30f;
-    int best_lanes = b->lanes;
-    bool best_pers = b->persistent, best_deg = b->by_degree;
     int rc = DG_OK;
-    bool best_bins = b->xcd_bins;
     for (int p = 1; p <= PMAX_GPU && !rc; p++) rc = dg_batch_set_degree_lanes(b, p, 0);
     // each size with both orders in the three launch forms (the all-degree
-    // kernel with one workgroup per tile or persistent, one launch per degree)
-    for (int cand = 0; cand < 18 && !rc; cand++) {
-        const int l = cands[cand / 6], form = (cand % 6) >> 1, bins = cand & 1;
+    // kernel with one workgroup per tile or persistent, one launch per degree),
+    // timed round-robin over three rounds, each keeping its fastest run (a
+    // slow stretch of the device then costs every candidate alike)
+    struct Cand { int lanes, form, bins; float t; };
+    std::vector<Cand> cl;
+    for (int cand = 0; cand < 18; cand++) {
+        const int form = (cand % 6) >> 1;
         if (batch_degree_only(b) && form != 2) continue;   // shock capturing, SD: by degree only
-        if ((rc = dg_batch_set_options(b, b->use_graph, l))) break;
-        if ((rc = dg_batch_set_by_degree(b, form == 2))) break;
-        if ((rc = dg_batch_set_xcd_bins(b, bins))) break;
-        if (form == 1 && b->pgrid < 1) continue;
-        if ((rc = dg_batch_set_persistent(b, form == 1))) break;
-        float ms;
-        if ((rc = time_steps(&ms))) break;
-        if (ms < best) {
-            best = ms; best_lanes = l; best_pers = b->persistent; best_bins = b->xcd_bins; best_deg = b->by_degree;
-        }
+        cl.push_back({cands[cand / 6], form, cand & 1, 1e30f});
     }
+    for (int round = 0; round < 3 && !rc; round++)
+        for (Cand& cd : cl) {
+            if ((rc = dg_batch_set_options(b, b->use_graph, cd.lanes))) break;
+            if ((rc = dg_batch_set_by_degree(b, cd.form == 2))) break;
+            if ((rc = dg_batch_set_xcd_bins(b, cd.bins))) break;
+            if (cd.form == 1 && b->pgrid < 1) continue;   // no resident grid at this size
+            if ((rc = dg_batch_set_persistent(b, cd.form == 1))) break;
+            float ms;
+            if ((rc = time_steps(&ms))) break;
+            cd.t = std::min(cd.t, ms);
+        }
+    float best = 1e30f;
+    int best_lanes = b->lanes;
+    bool best_pers = b->persistent, best_deg = b->by_degree, best_bins = b->xcd_bins;
+    for (const Cand& cd : cl)
+        if (cd.t < best) {
+            best = cd.t; best_lanes = cd.lanes; best_pers = cd.form == 1; best_deg = cd.form == 2;
+            best_bins = cd.bins != 0;
+        }
     int rc2 = dg_batch_set_options(b, b->use_graph, best_lanes);
     if (!rc2) rc2 = dg_batch_set_by_degree(b, best_deg);
     if (!rc2) rc2 = dg_batch_set_xcd_bins(b, best_bins);

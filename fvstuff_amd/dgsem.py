@@ -417,9 +417,11 @@ class Batch:
 
     def autotune(self, Qs, cfl):
         """Time the workgroup sizes and orders in the launch forms (by degree
-        only for a batch with AV / limiter), then each degree's own size, and
-        keep the fastest; returns the common lanes per workgroup (see
-        options(), by_degree() and degree_lanes() for the schedule)."""
+        only for a batch with AV / limiter or the SD scheme), round-robin over
+        three rounds, then each degree's own size, and keep the fastest; Qs
+        are advanced (9 steps per timing).  Returns the common lanes per
+        workgroup (see options(), by_degree() and degree_lanes() for the
+        schedule)."""
         cfl = [float(cfl)] * self.n if isinstance(cfl, (int, float)) else [float(c) for c in cfl]
         g = C.c_int()
         self._check(self.lib.dg_batch_autotune(self._b, self._ptrs(Qs), (C.c_double * self.n)(*cfl), C.byref(g)))

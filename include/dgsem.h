@@ -355,11 +355,14 @@ int         dg_batch_get_degree_lanes(const dg_batch* b, int p, int* lanes);
 int         dg_batch_get_options(const dg_batch* b, int* use_graph, int* lanes, int* persistent, int* xcd_bins);
 /* Time the three workgroup sizes, each with both orders in the three launch
  * forms (all-degree kernel one workgroup per tile, persistent, by degree; a
- * batch with AV / limiter: the by-degree form only), on the given states
- * (they are advanced by a few steps) and keep the fastest;
- * in the by-degree form then refine each degree's workgroup size on its own
- * (kept if the step runs >= 1 % faster).  Writes the common size to *lanes if
- * non-NULL. */
+ * batch with AV / limiter or the SD scheme: the by-degree form only), on the
+ * given states, round-robin over three rounds (each timing: 3 untimed steps
+ * that rebuild the graph, then 6 timed steps; a candidate keeps its fastest
+ * timing), and keep the fastest; in the by-degree form then refine each
+ * degree's workgroup size on its own (kept if the step runs >= 1 % faster).
+ * The states are advanced by 9 steps per timing at cfl[i] (at most 18 x 3
+ * timings, plus 1 + 2 per degree present).  Writes the common size to *lanes
+ * if non-NULL. */
 int         dg_batch_autotune(dg_batch* b, double* const* Qs, const double* cfl, int* lanes);
 
 #ifdef __cplusplus
